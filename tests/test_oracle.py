@@ -70,7 +70,7 @@ def test_merge_network_loop_keeps_low_block_exact():
             outs.append(a + b)
     g = np.array(outs, dtype=np.uint32)
     lo = oracle.network_u32(4, g.reshape(-1)).reshape(-1, 8)
-    zeros = 8 - g.sum(1)
+    zeros = 8 - g.sum(1).astype(np.int64)
     assert np.all(lo[:, :3] <= lo[:, 1:4])
     assert np.array_equal(lo[:, :4].sum(1), np.maximum(0, 4 - zeros))  # the 4 smallest
     full = oracle.network_u32(5, g.reshape(-1)).reshape(-1, 8)
