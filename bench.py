@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Headline benchmark: plain ascending sort of 2^28 uint32 keys (1 GiB), raw
+std::mt19937(42) output, on one MI355X through libogs.so's C-ABI.
+
+A step is one full sort of the same resident input (ogs_sort_keep reads the
+pristine input buffer and ping-pongs between two work buffers, so no restore
+copy runs inside the timed region).  Prints one JSON line:
+  value        whole-job G keys/s over the K timed steps
+  roofline     the dominant kernel (by summed time) vs the 8 TB/s HBM peak,
+               algorithmic bytes per launch (2 * n * w) / its mean launch time,
+               timed with HIP events recorded on the launching stream
+  pass_roofline the pass-count roofline of the whole sort: P * 2 * n * w / t
+  cpu_baseline the oracle (a C restatement of Origami's mergesort) on a bounded
+               sample of the same input on this host's cores
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "origami-sort_amd"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    calibrated against a copy kernel), committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(kind_oracle, host_keys, sample_n, threads):
+    import numpy as np
+
+    import oracle
+    sample = np.ascontiguousarray(host_keys[:sample_n])
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = oracle.sort(kind_oracle, sample, threads=threads, min_k=2)
+        times.append(time.perf_counter() - t0)
+    assert bool((out[1:] >= out[:-1]).all())
+    t = sorted(times)[1]
+    return {"value": round(sample_n / t / 1e9, 6), "unit": "G keys/s", "cores": threads, "kind": "port",
+            "sample": f"first 2^{sample_n.bit_length() - 1} keys of the same input, median of 3, "
+                      f"oracle/origami_oracle.c sort_multi_thread restatement, {threads} threads, min_k=2"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--log2n", type=int, default=28)
+    ap.add_argument("--kind", default="u32", choices=["u32", "f32", "u64", "kv_u64u64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              "the sort shards by one all-to-all after local sorts, which is design only here.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ogs
+    kind = ogs.KINDS[args.kind]
+    n = 1 << args.log2n
+    w = ogs.WIDTH[kind]
+    dist = ogs.NORMAL if kind == ogs.F32 else ogs.UNIFORM
+    log(f"bench: generating {n} {args.kind} keys (seed 42)")
+    host = ogs.generate(kind, dist, n, seed=42)
+    ctx = ogs.Context(0)
+    d_in = ogs.to_device(host)
+    d_a = torch.empty_like(d_in)
+    d_b = torch.empty_like(d_in)
+    torch.cuda.synchronize()
+
+    def step():
+        return ctx.sort_keep_ptr(kind, d_in.data_ptr(), d_a.data_ptr(), d_b.data_ptr(), n)
+
+    log(f"bench: warmup {args.warmup}")
+    res = None
+    for _ in range(args.warmup):
+        res = step()
+    torch.cuda.synchronize()
+    # size-independent checks of the result: sorted (in the key order) and the
+    # same multiset (sum and xor of the raw bytes as 64-bit words)
+    out_t = d_a if res == d_a.data_ptr() else d_b
+    if args.kind == "u32":
+        o = out_t.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+        ok_sorted = bool((o[1:] >= o[:-1]).all())
+        ok_sum = int(o.sum()) == int((d_in.view(torch.int32).to(torch.int64) & 0xFFFFFFFF).sum())
+        log(f"bench: result sorted={ok_sorted} multiset-sum={ok_sum}")
+        assert ok_sorted and ok_sum
+        del o
+
+    ctx.profile(True)
+    ctx.profile_reset()
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.steps} steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms_step / 1e3) / 1e9
+
+    kernels = {}
+    for kid, name in ((0, "block_sort"), (1, "merge_partition"), (2, "merge_pass")):
+        ms, cnt, items = ctx.profile_read(kid)
+        if cnt:
+            kernels[name] = {"total_ms": ms, "launches": cnt, "mean_ms": ms / cnt, "items": items}
+    dom = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    dk = kernels[dom]
+    bytes_per_launch = 2.0 * (dk["items"] / dk["launches"]) * w
+    achieved = bytes_per_launch / (dk["mean_ms"] / 1e3) / 1e9
+    P = ctx.pass_count(kind, n)
+    pass_gbs = P * 2.0 * n * w / (ms_step / 1e3) / 1e9
+    line = {
+        "metric": "G keys/s + % HBM roofline, 2^28 u32 sort; batched keys/s and p50/p99 vs plain",
+        "value": round(value, 4),
+        "unit": "G keys/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"u32": "u32", "f32": "f32", "u64": "u64", "kv_u64u64": "u64+u64"}[args.kind],
+        "data": "synthetic: raw std::mt19937(42) keys" if kind == ogs.U32 else "synthetic: seeded generator",
+        "config": {"workload": f"plain sort, 2^{args.log2n} {args.kind} keys uniform, ascending, key-only",
+                   "n": n, "item_bytes": w},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(dom)},
+        "pass_roofline": {"P": P, "bytes_per_key": P * 2 * w, "achieved_GBs": round(pass_gbs, 2),
+                          "frac": round(pass_gbs / HBM_PEAK_GBS, 4),
+                          "single_pass_GBs": round(2.0 * n * w / (ms_step / 1e3) / 1e9, 2)},
+        "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                    for k, v in kernels.items()},
+    }
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        import oracle
+        okind = {"u32": oracle.U32, "f32": oracle.F32, "u64": oracle.U64, "kv_u64u64": oracle.KV_U64U64}[args.kind]
+        log(f"bench: cpu baseline on {threads} threads")
+        line["cpu_baseline"] = cpu_baseline(okind, host, min(n, 1 << 26), threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

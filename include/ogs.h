@@ -1,0 +1,150 @@
+/*
+ * ogs.h -- C-ABI of the MI355X (gfx950) Origami mergesort library (libogs.so).
+ *
+ * The reference (arif-arman/origami-sort) is a header-only C++ template
+ * library whose sort entry points are
+ *   origami_sorter::sort_single_thread(Item* d, Item* tmp, Item* end, ui64 sort_n,
+ *                                      ui min_k, Item* interim)   Origami/sorter.h:3851-3852
+ *   origami_sorter::sort_multi_thread(Item* d, Item* out, ui64 n, ui n_threads,
+ *                                     ui n_cores, ui min_k, Item* kway_buf)
+ *                                                                  Origami/sorter.h:4194-4195
+ * with the item type fixed at compile time (Origami/config.h:23-24): uint32,
+ * signed int64, or a packed 16-byte {int64 key, int64 value}
+ * (Origami/commons.h:51-72).  This ABI takes the same shapes at run time and
+ * adds the types the reference lacks (int32, uint64, float32, unsigned KV).
+ * include/origami_gpu_compat.h re-creates the two template entry points on
+ * top of it, so reference host code recompiles unchanged.
+ *
+ * Conventions
+ *  - Device pointers unless a name says _host.  Buffers hold n items; items
+ *    are 4 bytes (U32/I32/F32), 8 bytes (U64/I64) or 16 bytes (KV_*: key at
+ *    offset 0, value at offset 8, like KeyValue<i64,i64>).
+ *  - Sorting is ascending.  The result lands in *sorted, which is d_data or
+ *    d_scratch (the other one is clobbered), as the reference returns d or
+ *    tmp (Origami/merger.h:4225, Origami/sorter.h:3909-3914).
+ *  - Key-only results are the unique sorted permutation.  KV sorts are not
+ *    stable (neither is the reference): keys come out sorted and each key
+ *    keeps its multiset of values.
+ *  - F32 keys sort in IEEE-754 totalOrder (-NaN < -inf < ... < -0 < +0 < ... <
+ *    +inf < +NaN).  I32/I64/KV_I64I64 keys are signed; U32/U64/KV_U64U64 are
+ *    unsigned.
+ *  - Every call returns OGS_OK (0) or a negative ogs_status; the message of
+ *    the last failure on the calling thread is ogs_last_error().  No call
+ *    exits the process (the reference's ReportError does, commons.h:104).
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Calls are stream-ordered; a context may be shared by threads that use
+ *    different streams.
+ */
+#ifndef OGS_H
+#define OGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    OGS_U32 = 0,        /* uint32  (Origami KEY_TYPE 0, config.h:23) */
+    OGS_I32 = 1,        /* int32   (no reference counterpart) */
+    OGS_U64 = 2,        /* uint64  (no reference counterpart) */
+    OGS_I64 = 3,        /* int64   (Origami KEY_TYPE 1) */
+    OGS_F32 = 4,        /* float32 (no reference counterpart) */
+    OGS_KV_U64U64 = 5,  /* {uint64 key, uint64 value} (unsigned form of KEY_TYPE 2) */
+    OGS_KV_I64I64 = 6   /* {int64 key, int64 value}   (Origami KEY_TYPE 2) */
+} ogs_key_t;
+
+typedef enum {
+    OGS_OK = 0,
+    OGS_ERR_ARG = -1,       /* bad argument (null pointer, unknown key type, ...) */
+    OGS_ERR_HIP = -2,       /* a HIP runtime call failed */
+    OGS_ERR_NOMEM = -3,     /* device or host allocation failed / over budget */
+    OGS_ERR_STATE = -4,     /* call not valid in the object's current state */
+    OGS_ERR_TIMEOUT = -5    /* a wait ran past its deadline */
+} ogs_status;
+
+typedef struct ogs_ctx ogs_ctx;
+
+/* Context: binds a device, owns pooled workspace (co-rank tables, staging).
+ * Replaces the reference's process globals (Origami/sorter.h:3941-3959). */
+int ogs_ctx_create(ogs_ctx **ctx, int device);
+int ogs_ctx_destroy(ogs_ctx *ctx);
+
+/* Bytes per item of a key type (0 for an unknown type). */
+size_t ogs_item_size(ogs_key_t kt);
+
+/* Full sort of n items in d_data, d_scratch = n items of scratch.
+ * Replaces sort_single_thread / sort_multi_thread (Origami/sorter.h:3851,4194)
+ * for sort_n == n; any n >= 0 (the reference needs a power of two >= P2_N). */
+int ogs_sort(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch, uint64_t n,
+             void **d_sorted, void *stream);
+
+/* Out-of-place form: reads n items from d_in (left intact) and sorts them
+ * using d_data and d_scratch (n items each) as ping-pong buffers; the result
+ * lands in *d_sorted (d_data or d_scratch).  The reference always sorts in
+ * place (sort_single_thread's d, sorter.h:3851); this form serves callers
+ * that must keep their input (ORDER BY over a table column). */
+int ogs_sort_keep(ogs_ctx *ctx, ogs_key_t kt, const void *d_in, void *d_data, void *d_scratch,
+                  uint64_t n, void **d_sorted, void *stream);
+
+/* Sorts every chunk of `chunk` items independently (the last may be shorter):
+ * the reference's sort_single_thread with sort_n < n (Origami/sorter.h:3857-3868,
+ * and the chunked benchmark Origami/bench_sort_phases.cpp:1307-1425). */
+int ogs_sort_chunks(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch, uint64_t n,
+                    uint64_t chunk, void **d_sorted, void *stream);
+
+/* Sorts each CSR segment [d_offsets[i], d_offsets[i+1]) of d_data
+ * independently, all segments in one batched launch sequence.  d_offsets is a
+ * device array of nseg+1 non-decreasing uint64 offsets with d_offsets[0] == 0
+ * and d_offsets[nseg] == n (the total item count, given in n).  No reference
+ * counterpart beyond uniform chunks (see ogs_sort_chunks). */
+int ogs_segmented_sort(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch,
+                       const uint64_t *d_offsets, uint64_t nseg, uint64_t n,
+                       void **d_sorted, void *stream);
+
+/* Host-memory convenience: copies h_data (n items) to the device, sorts, and
+ * copies the result back into h_data; synchronous.  This is what
+ * origami_gpu_compat.h calls for the reference's host-pointer signatures. */
+int ogs_sort_host(ogs_ctx *ctx, ogs_key_t kt, void *h_data, uint64_t n, uint64_t chunk);
+
+/* Analytic HBM pass count P of a plain sort of n items of type kt with the
+ * context's current tiles (the roofline denominator: bytes = P * 2 * n * w). */
+int ogs_pass_count(ogs_ctx *ctx, ogs_key_t kt, uint64_t n);
+
+/* Seeded input generator (the datagen::Writer analogue, Origami/writer.h:47-242,
+ * but reproducible): dist 0 uniform (raw std::mt19937 for 4-byte kinds,
+ * std::mt19937_64 otherwise), 1 Zipf(s=1.1) over 2^20 ranks, 2 sorted,
+ * 3 reverse sorted, 4 sixteen distinct values, 5 N(0,1) float32 with -0.0 at
+ * index 0 and +0.0 at n/2.  KV values are the original index.  Host memory. */
+int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h_out);
+
+/* Tile table (the run-time counterpart of the tuned constants in
+ * Origami/config.h:45-264).  param: "plain_block", "plain_merge",
+ * "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge"; block
+ * and merge parameters take an index into ogs_shape_info's lists,
+ * seg_small_max an item count (0 = the small shape's tile). */
+int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value);
+int ogs_tune_get(ogs_ctx *ctx, ogs_key_t kt, const char *param, int *value);
+
+/* Compiled kernel shapes of a key type: merge = 0 lists block-sort shapes
+ * (workgroup size, items per lane, in-wave bitonic levels), merge = 1 merge
+ * shapes (wlev = -1).  Fills entry idx when it exists; returns the count. */
+int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *wlev);
+
+/* Per-kernel timing with HIP events recorded on the launching stream, for
+ * plain sorts issued while enabled.  kind: 0 block sort, 1 merge partition,
+ * 2 merge pass.  read returns the summed milliseconds, the launch count and
+ * the items those launches covered (waits for the recorded events). */
+int ogs_profile_enable(ogs_ctx *ctx, int on);
+int ogs_profile_read(ogs_ctx *ctx, int kind, double *total_ms, uint64_t *launches, uint64_t *items);
+int ogs_profile_reset(ogs_ctx *ctx);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *ogs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OGS_H */

@@ -333,7 +333,8 @@ static T *sort_##S(T *d, T *tmp, uint64_t n, uint64_t sort_n, int p1n, uint64_t 
                 partition_##S(rp, rl, K, r1, sp1);                                            \
                 const T *sub[64]; uint64_t sl[64];                                            \
                 for (int k = 0; k < K; k++) { sub[k] = rp[k] + sp0[k]; sl[k] = sp1[k] - sp0[k]; }\
-                kway_##S(sub, sl, K, dst + s0 + r0, 1024); }                                  \
+                if (K == 2) mergebl_##S(sub[0], sl[0], sub[1], sl[1], dst + s0 + r0, nreg);      \
+                else kway_##S(sub, sl, K, dst + s0 + r0, 1024); }                             \
             free(gs); free(ge); }                                                             \
         T *t = src; src = dst; dst = t; run = span; }                                         \
     return src; }

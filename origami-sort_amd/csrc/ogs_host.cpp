@@ -1,0 +1,155 @@
+// ogs_host.cpp -- host-side parts of the C-ABI that launch nothing: the seeded
+// input generator (the datagen::Writer analogue, Origami/writer.h:47-242) and
+// the tile-table accessors used by the autotuner.
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+#include "ogs_internal.hpp"
+
+using namespace ogs;
+
+namespace {
+
+uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+const char *kParamNames[TP_COUNT] = {"plain_block", "plain_merge", "seg_small_block",
+                                     "seg_small_max", "seg_large_block", "seg_merge"};
+
+int param_index(const char *name) {
+    if (!name) return -1;
+    for (int i = 0; i < TP_COUNT; i++)
+        if (!strcmp(name, kParamNames[i])) return i;
+    return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Distributions as documented in include/ogs.h.  4-byte kinds draw raw
+// std::mt19937 output for the uniform case (the 2^20 / 2^28 u32 workloads);
+// every other case draws std::mt19937_64 and narrows to the top 32 bits.
+// The engines' outputs are fixed by the C++ standard, so any conforming
+// library reproduces these bytes.
+int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h_out) {
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    if (dist < 0 || dist > 5) return fail(OGS_ERR_ARG, "unknown distribution %d", dist);
+    if (n && !h_out) return fail(OGS_ERR_ARG, "h_out is NULL");
+    const int w = ki.width;
+    if (w == 4 && dist == 0) {
+        std::mt19937 g((uint32_t)seed);
+        uint32_t *o = (uint32_t *)h_out;
+        for (uint64_t i = 0; i < n; i++) o[i] = (uint32_t)g();
+        return OGS_OK;
+    }
+    std::mt19937_64 g(seed);
+    std::vector<double> cdf;
+    const uint64_t R = 1u << 20;
+    if (dist == 1) {
+        cdf.resize(R);
+        double acc = 0;
+        for (uint64_t r = 0; r < R; r++) {
+            acc += std::pow((double)(r + 1), -1.1);
+            cdf[r] = acc;
+        }
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        uint64_t key = 0;
+        uint64_t raw = (dist == 0 || dist == 1 || dist == 4 || dist == 5) ? g() : 0;
+        switch (dist) {
+        case 0: key = raw; break;
+        case 1: {
+            const double u = (double)(raw >> 11) * (1.0 / 9007199254740992.0) * cdf[R - 1];
+            key = mix64((uint64_t)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin()));
+        } break;
+        case 2: key = (uint64_t)i * (UINT64_MAX / n); break;
+        case 3: key = (uint64_t)(n - 1 - i) * (UINT64_MAX / n); break;
+        case 4: key = mix64(raw & 15u); break;
+        case 5: {
+            const double u1 = ((double)(raw >> 11) + 1.0) * (1.0 / 9007199254740992.0);
+            const double u2 = (double)((raw * 0x9E3779B97F4A7C15ull) >> 11) * (1.0 / 9007199254740992.0);
+            const float f = (float)(std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2));
+            uint32_t b;
+            memcpy(&b, &f, 4);
+            key = b;
+        } break;
+        }
+        if (w == 4) {
+            uint32_t k32 = dist == 5 ? (uint32_t)key : (uint32_t)(key >> 32);
+            if (dist == 2 || dist == 3) k32 = (uint32_t)((uint64_t)(dist == 2 ? i : n - 1 - i) * (UINT32_MAX / n));
+            ((uint32_t *)h_out)[i] = k32;
+        } else if (w == 8) {
+            ((uint64_t *)h_out)[i] = key;
+        } else {
+            ((uint64_t *)h_out)[2 * i] = key;
+            ((uint64_t *)h_out)[2 * i + 1] = i;
+        }
+    }
+    if (w == 4 && dist == 5 && n) {
+        ((uint32_t *)h_out)[0] = 0x80000000u;
+        ((uint32_t *)h_out)[n / 2] = 0u;
+    }
+    return OGS_OK;
+}
+
+// Tile-table access (no reference counterpart beyond the compile-time
+// constants of Origami/config.h).  `param` is one of plain_block, plain_merge,
+// seg_small_block, seg_small_max, seg_large_block, seg_merge.
+int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
+    if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    const int p = param_index(param);
+    if (p < 0) return fail(OGS_ERR_ARG, "unknown tuning parameter '%s'", param ? param : "(null)");
+    int cnt = 0;
+    if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
+    else if (p != TP_SEG_SMALL_MAX) block_shapes(ki.kc, &cnt);
+    if (p == TP_SEG_SMALL_MAX ? value < 0 : (value < 0 || value >= cnt))
+        return fail(OGS_ERR_ARG, "value %d out of range for '%s'", value, param);
+    reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
+    return OGS_OK;
+}
+
+int ogs_tune_get(ogs_ctx *ctx, ogs_key_t kt, const char *param, int *value) {
+    if (!ctx || !value) return fail(OGS_ERR_ARG, "NULL argument");
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    const int p = param_index(param);
+    if (p < 0) return fail(OGS_ERR_ARG, "unknown tuning parameter '%s'", param ? param : "(null)");
+    *value = reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p];
+    return OGS_OK;
+}
+
+// Compiled shapes: merge = 0 lists block-sort shapes (block, ipt, wave levels),
+// merge = 1 merge shapes (block, ipt, wave levels = -1).  Returns the count.
+int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *wlev) {
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    int cnt = 0;
+    if (merge) {
+        const MergeShapeFns *t = merge_shapes(ki.kc, &cnt);
+        if (idx >= 0 && idx < cnt) {
+            if (block) *block = t[idx].block;
+            if (ipt) *ipt = t[idx].ipt;
+            if (wlev) *wlev = -1;
+        }
+    } else {
+        const BlockShapeFns *t = block_shapes(ki.kc, &cnt);
+        if (idx >= 0 && idx < cnt) {
+            if (block) *block = t[idx].block;
+            if (ipt) *ipt = t[idx].ipt;
+            if (wlev) *wlev = t[idx].wlev;
+        }
+    }
+    return cnt;
+}
+
+}  // extern "C"

@@ -1,0 +1,94 @@
+// ogs_internal.hpp -- host-side objects behind the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/ogs.h"
+#include "sort_kernels.hpp"
+
+namespace ogs {
+
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+
+struct KeyInfo {
+    KeyClass kc;
+    int width;  // bytes per item
+    int codec;  // 0 identity, 1 sign flip, 2 float totalOrder
+};
+bool key_info(ogs_key_t kt, KeyInfo *ki);
+
+// Tile table per key class: indices into the compiled shape lists (the
+// analogue of the tuned constants in Origami/config.h:45-264).
+enum TuneParam {
+    TP_PLAIN_BLOCK = 0,     // block-sort shape of plain sorts
+    TP_PLAIN_MERGE = 1,     // merge shape of plain sorts
+    TP_SEG_SMALL_BLOCK = 2, // block-sort shape of the small segment class
+    TP_SEG_SMALL_MAX = 3,   // longest segment in the small class (0 = its tile)
+    TP_SEG_LARGE_BLOCK = 4, // block-sort shape of segments above the small class
+    TP_SEG_MERGE = 5,       // merge shape of segmented passes
+    TP_COUNT = 6
+};
+struct Tuning {
+    int v[TP_COUNT];
+};
+
+// Device workspace bound to one stream (co-rank tables, descriptors).  Work
+// on one stream is ordered, so reusing the block across calls on the same
+// stream is safe; growing it frees the old block (hipFree synchronises).
+struct Workspace {
+    void *dev = nullptr;
+    size_t cap = 0;
+    std::vector<uint8_t> host;  // staging for descriptor uploads
+    int reserve(size_t bytes);
+    ~Workspace();
+};
+
+// Per-kernel timing with HIP events recorded on the launching stream
+// (ogs_profile_*): kernel kinds 0 block sort, 1 merge partition, 2 merge pass.
+enum ProfKind { PK_BLOCK = 0, PK_PARTITION = 1, PK_MERGE = 2, PK_COUNT = 3 };
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+        uint64_t items;
+        bool own_a;  // false when `a` is another record's end event
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> pool;
+    double total_ms[PK_COUNT] = {0, 0, 0};
+    uint64_t count[PK_COUNT] = {0, 0, 0};
+    uint64_t items[PK_COUNT] = {0, 0, 0};
+    hipEvent_t get();
+    int flush();  // waits for pending events and accumulates
+    void reset();
+};
+
+struct Ctx {
+    int device = 0;
+    Profiler prof;
+    Tuning tune[3];
+    std::mutex mu;  // guards ws
+    std::map<hipStream_t, Workspace *> ws;
+    std::mutex host_mu;  // serialises ogs_sort_host
+    void *host_buf[2] = {nullptr, nullptr};
+    size_t host_cap = 0;
+
+    Workspace *workspace(hipStream_t s);
+    const BlockShapeFns &block_shape(KeyClass kc, int param);
+    const MergeShapeFns &merge_shape(KeyClass kc, int param, uint64_t block_tile);
+    // in == nullptr: sort data in place (data/scratch ping-pong); otherwise read
+    // the input from `in` (left intact) and ping-pong between data and scratch.
+    int sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s);
+    int segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg, void **out,
+                  hipStream_t s);
+    int pass_count(ogs_key_t kt, uint64_t n);
+};
+
+}  // namespace ogs

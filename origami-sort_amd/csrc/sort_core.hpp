@@ -1,0 +1,273 @@
+// sort_core.hpp -- device building blocks of the gfx950 mergesort.
+//
+// Reference pipeline (arif-arman/origami-sort, Origami/):
+//   phase1_sort / in_register_sort*  (sorter.h:33-1556, 3405-3828)  -> per-lane network + wave bitonic
+//   mergebl2_* / merge_series        (merger.h:45-4228)             -> merge-path co-rank merges in LDS
+//   Partition_Ptrs                   (Partition.h:117-166)          -> co-rank searches (2-way and k-way)
+//   origami_utils::store<stream>     (merge_utils.h:61-81)          -> coalesced / non-temporal stores
+//
+// Keys travel in an "ordered" unsigned form (u32 or u64) whose unsigned order is
+// the requested order; the codec is applied on the first load and undone on the
+// last store, so signed and float keys cost no extra pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ogs {
+
+// ----------------------------------------------------------------------------
+// Key codecs.  0: identity, 1: flip the sign bit (two's-complement order),
+// 2: float32 IEEE totalOrder.  The reference compares u32 unsigned
+// (avx2_utils.h:129-133) and i64 signed (avx2_utils.h:140-145).
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t enc_key(uint32_t x, int c) {
+    if (c == 1) return x ^ 0x80000000u;
+    if (c == 2) return x ^ ((uint32_t)((int32_t)x >> 31) | 0x80000000u);
+    return x;
+}
+__device__ __forceinline__ uint32_t dec_key(uint32_t x, int c) {
+    if (c == 1) return x ^ 0x80000000u;
+    if (c == 2) return x ^ (~(uint32_t)((int32_t)x >> 31) | 0x80000000u);
+    return x;
+}
+__device__ __forceinline__ uint64_t enc_key(uint64_t x, int c) { return c == 1 ? x ^ (1ull << 63) : x; }
+__device__ __forceinline__ uint64_t dec_key(uint64_t x, int c) { return c == 1 ? x ^ (1ull << 63) : x; }
+
+template <typename K> struct KeyMax;
+template <> struct KeyMax<uint32_t> { static constexpr uint32_t v = 0xFFFFFFFFu; };
+template <> struct KeyMax<uint64_t> { static constexpr uint64_t v = ~0ull; };
+
+// ----------------------------------------------------------------------------
+// Global memory items.  Key-only arrays hold K; KV arrays hold 16-byte
+// {u64 key, u64 value} records (KeyValue<i64,i64>, commons.h:51-72).
+// ----------------------------------------------------------------------------
+template <typename K, bool HV>
+struct GMem {
+    static __device__ __forceinline__ void load(const void *p, uint64_t i, K &k, uint64_t &v) {
+        if constexpr (HV) {
+            const ulonglong2 *q = (const ulonglong2 *)p;
+            ulonglong2 t = q[i];
+            k = (K)t.x;
+            v = t.y;
+        } else {
+            k = ((const K *)p)[i];
+            (void)v;
+        }
+    }
+    static __device__ __forceinline__ K load_key(const void *p, uint64_t i) {
+        if constexpr (HV) return (K)((const ulonglong2 *)p)[i].x;
+        else return ((const K *)p)[i];
+    }
+    static __device__ __forceinline__ void store(void *p, uint64_t i, K k, uint64_t v) {
+        if constexpr (HV) {
+            ulonglong2 t;
+            t.x = (unsigned long long)k;
+            t.y = v;
+            ((ulonglong2 *)p)[i] = t;
+        } else {
+            ((K *)p)[i] = k;
+            (void)v;
+        }
+    }
+};
+
+// LDS index padding: one spare slot every 32 banks' worth of keys keeps the
+// strided per-lane streams of the merge path on distinct banks.
+template <typename K> __device__ __forceinline__ int lpad(int i) {
+    if constexpr (sizeof(K) == 4) return i + (i >> 5);
+    else return i + (i >> 4);
+}
+template <typename K> constexpr int lpad_size(int n) {
+    return sizeof(K) == 4 ? n + (n >> 5) + 2 : n + (n >> 4) + 2;
+}
+
+// ----------------------------------------------------------------------------
+// Compare-exchange (SWAP / SWAPKV2, commons.h:111-149): a <- smaller, b <- larger.
+// ----------------------------------------------------------------------------
+template <bool HV, typename K>
+__device__ __forceinline__ void cex(K &a, K &b, uint64_t &va, uint64_t &vb) {
+    if constexpr (!HV) {
+        K lo = a < b ? a : b;
+        b = a < b ? b : a;
+        a = lo;
+        (void)va; (void)vb;
+    } else {
+        bool sw = b < a;
+        K ta = sw ? b : a, tb = sw ? a : b;
+        uint64_t ua = sw ? vb : va, ub = sw ? va : vb;
+        a = ta; b = tb; va = ua; vb = ub;
+    }
+}
+
+// Batcher odd-even merge sort network over N registers, fully unrolled
+// (the reference uses fixed tables, e.g. in_register_sort8/16, sorter.h:40-66).
+template <int N, bool HV, typename K>
+__device__ __forceinline__ void lane_network(K (&r)[N], uint64_t (&v)[N]) {
+#pragma unroll
+    for (int p = 1; p < N; p <<= 1)
+#pragma unroll
+        for (int k = p; k >= 1; k >>= 1)
+#pragma unroll
+            for (int j = k % p; j + k < N; j += 2 * k)
+#pragma unroll
+                for (int i = 0; i < k; i++)
+                    if (i + j + k < N && (i + j) / (2 * p) == (i + j + k) / (2 * p))
+                        cex<HV>(r[i + j], r[i + j + k], v[i + j], v[i + j + k]);
+}
+
+// ----------------------------------------------------------------------------
+// Cross-lane exchange with a constant XOR mask.  DPP quad_perm for 1/2/3,
+// DPP row_half_mirror (xor 7) and row_mirror (xor 15), ds_swizzle bitmask mode
+// for the other masks below 32, ds_bpermute beyond.
+// ----------------------------------------------------------------------------
+template <int M>
+__device__ __forceinline__ uint32_t xshfl32(uint32_t x) {
+    if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
+    else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
+    else if constexpr (M == 3) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xF, 0xF, false);
+    else if constexpr (M == 7) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, false);
+    else if constexpr (M == 15) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xF, 0xF, false);
+    else if constexpr (M < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (M << 10) | 0x1F);
+    else return (uint32_t)__shfl_xor((int)x, M, 64);
+}
+template <int M> __device__ __forceinline__ uint32_t xshfl(uint32_t x) { return xshfl32<M>(x); }
+template <int M> __device__ __forceinline__ uint64_t xshfl(uint64_t x) {
+    uint32_t lo = xshfl32<M>((uint32_t)x), hi = xshfl32<M>((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One exchange step between lane pairs: the lane whose `upper` flag is clear
+// keeps the smaller item, its partner the larger; on equal keys both keep their
+// own item, so no item is duplicated or lost.
+template <bool HV, typename K>
+__device__ __forceinline__ void lane_pair_keep(K &mine, uint64_t &vm, K other, uint64_t vo, bool upper) {
+    bool keep = upper ? !(mine < other) : !(other < mine);
+    mine = keep ? mine : other;
+    if constexpr (HV) vm = keep ? vm : vo;
+}
+
+// In-lane bitonic half-cleaners: distances IPT/2 .. 1.
+template <int IPT, bool HV, typename K>
+__device__ __forceinline__ void lane_cleaners(K (&r)[IPT], uint64_t (&v)[IPT]) {
+#pragma unroll
+    for (int d = IPT / 2; d >= 1; d >>= 1)
+#pragma unroll
+        for (int i = 0; i < IPT; i++)
+            if (!(i & d)) cex<HV>(r[i], r[i + d], v[i], v[i + d]);
+}
+
+// Cross-lane half-cleaner at lane distance D, then the smaller distances.
+template <int D, int IPT, bool HV, typename K>
+__device__ __forceinline__ void cross_cleaners(K (&r)[IPT], uint64_t (&v)[IPT], int lane) {
+    if constexpr (D >= 1) {
+        const bool up = (lane & D) != 0;
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            K o = xshfl<D>(r[i]);
+            uint64_t vo = 0;
+            if constexpr (HV) vo = xshfl<D>(v[i]);
+            lane_pair_keep<HV>(r[i], v[i], o, vo, up);
+        }
+        cross_cleaners<D / 2, IPT, HV>(r, v, lane);
+    }
+}
+
+// Merges sorted runs held by lane groups of G/2 lanes (lane-major: lane L holds
+// run positions L*IPT .. L*IPT+IPT-1) into runs held by G lanes.  Bitonic merge
+// in the "flip" form: first stage pairs item i of lane L with item IPT-1-i of
+// lane L^(G-1), then half-cleaners across lanes, then within each lane.  This is
+// the role of the reference's mcmerge / mrmerge matrix merges (sorter.h:1628-2319).
+template <int G, int IPT, bool HV, typename K>
+__device__ __forceinline__ void wave_merge_level(K (&r)[IPT], uint64_t (&v)[IPT], int lane) {
+    const bool upper = (lane & (G / 2)) != 0;
+    K pk[IPT];
+    uint64_t pv[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+        pk[i] = xshfl<G - 1>(r[IPT - 1 - i]);
+        if constexpr (HV) pv[i] = xshfl<G - 1>(v[IPT - 1 - i]);
+        else pv[i] = 0;
+    }
+#pragma unroll
+    for (int i = 0; i < IPT; i++) lane_pair_keep<HV>(r[i], v[i], pk[i], pv[i], upper);
+    cross_cleaners<G / 4, IPT, HV>(r, v, lane);
+    lane_cleaners<IPT, HV>(r, v);
+}
+
+template <int LEVELS, int IPT, bool HV, typename K>
+__device__ __forceinline__ void wave_sort_levels(K (&r)[IPT], uint64_t (&v)[IPT], int lane) {
+    if constexpr (LEVELS >= 1) wave_merge_level<2, IPT, HV>(r, v, lane);
+    if constexpr (LEVELS >= 2) wave_merge_level<4, IPT, HV>(r, v, lane);
+    if constexpr (LEVELS >= 3) wave_merge_level<8, IPT, HV>(r, v, lane);
+    if constexpr (LEVELS >= 4) wave_merge_level<16, IPT, HV>(r, v, lane);
+    if constexpr (LEVELS >= 5) wave_merge_level<32, IPT, HV>(r, v, lane);
+    if constexpr (LEVELS >= 6) wave_merge_level<64, IPT, HV>(r, v, lane);
+}
+
+// ----------------------------------------------------------------------------
+// Merge path in LDS (the co-rank form of mergebl2: merger.h:964-1169).
+// A = sk[a_off .. a_off+la), B = sk[b_off .. b_off+lb).  Returns the number of
+// A items among the first `diag` outputs; ties go to A (stable).
+// ----------------------------------------------------------------------------
+template <typename K>
+__device__ __forceinline__ int lds_corank(const K *sk, int a_off, int la, int b_off, int lb, int diag) {
+    int lo = diag > lb ? diag - lb : 0;
+    int hi = diag < la ? diag : la;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        K a = sk[lpad<K>(a_off + mid)];
+        K b = sk[lpad<K>(b_off + diag - 1 - mid)];
+        if (!(b < a)) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Serial branch-free merge of `cnt` (<= IPT) outputs into registers starting at
+// A index ai and B index bi.  Reads one LDS item per output.  `limit` bounds
+// the LDS index read ahead of an exhausted side.
+template <int IPT, bool HV, typename K>
+__device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv, int a_off, int la, int b_off,
+                                                 int lb, int ai, int bi, int cnt, int limit, K (&r)[IPT],
+                                                 uint64_t (&v)[IPT]) {
+    int ia = a_off + ai, ib = b_off + bi;
+    const int ea = a_off + la, eb = b_off + lb;
+    K ka = sk[lpad<K>(min(ia, limit))], kb = sk[lpad<K>(min(ib, limit))];
+    uint64_t va = 0, vb = 0;
+    if constexpr (HV) {
+        va = sv[lpad<K>(min(ia, limit))];
+        vb = sv[lpad<K>(min(ib, limit))];
+    }
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+        if (i < cnt) {
+            bool takeA = ib >= eb || (ia < ea && !(kb < ka));
+            r[i] = takeA ? ka : kb;
+            if constexpr (HV) v[i] = takeA ? va : vb;
+            ia += takeA;
+            ib += !takeA;
+            int nx = min(takeA ? ia : ib, limit);
+            K nk = sk[lpad<K>(nx)];
+            ka = takeA ? nk : ka;
+            kb = takeA ? kb : nk;
+            if constexpr (HV) {
+                uint64_t nv = sv[lpad<K>(nx)];
+                va = takeA ? nv : va;
+                vb = takeA ? vb : nv;
+            }
+        }
+    }
+}
+
+// Store IPT consecutive registers to LDS at item index base (padded layout).
+template <int IPT, bool HV, typename K>
+__device__ __forceinline__ void lds_put(K *sk, uint64_t *sv, int base, const K (&r)[IPT], const uint64_t (&v)[IPT]) {
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+        sk[lpad<K>(base + i)] = r[i];
+        if constexpr (HV) sv[lpad<K>(base + i)] = v[i];
+    }
+}
+
+}  // namespace ogs

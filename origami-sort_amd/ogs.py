@@ -1,0 +1,184 @@
+"""ctypes binding of libogs.so (include/ogs.h) for the tests, the bench and the
+serving loop.  Device memory comes from torch tensors (plumbing only: the
+sort itself runs in the HIP kernels of libogs.so).
+
+The library must be built (``build.sh``); loading fails loudly otherwise --
+there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libogs.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "ogs.h")
+
+U32, I32, U64, I64, F32, KV_U64U64, KV_I64I64 = range(7)
+KINDS = {"u32": U32, "i32": I32, "u64": U64, "i64": I64, "f32": F32,
+         "kv_u64u64": KV_U64U64, "kv_i64i64": KV_I64I64}
+KIND_NAMES = {v: k for k, v in KINDS.items()}
+WIDTH = {U32: 4, I32: 4, F32: 4, U64: 8, I64: 8, KV_U64U64: 16, KV_I64I64: 16}
+KV_DTYPE = np.dtype([("key", "<u8"), ("value", "<u8")])
+KV_I_DTYPE = np.dtype([("key", "<i8"), ("value", "<i8")])
+NP_DTYPE = {U32: np.dtype("<u4"), I32: np.dtype("<i4"), U64: np.dtype("<u8"), I64: np.dtype("<i8"),
+            F32: np.dtype("<f4"), KV_U64U64: KV_DTYPE, KV_I64I64: KV_I_DTYPE}
+UNIFORM, ZIPF, SORTED, REVERSE, DISTINCT16, NORMAL = range(6)
+
+_lib = None
+
+
+class OgsError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OgsError(f"{LIB_PATH} is not built; run origami-sort_amd/build.sh")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64, i, cp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p
+        L.ogs_ctx_create.argtypes = [ctypes.POINTER(vp), i]
+        L.ogs_ctx_destroy.argtypes = [vp]
+        L.ogs_item_size.argtypes = [i]
+        L.ogs_item_size.restype = ctypes.c_size_t
+        L.ogs_sort.argtypes = [vp, i, vp, vp, u64, ctypes.POINTER(vp), vp]
+        L.ogs_sort_chunks.argtypes = [vp, i, vp, vp, u64, u64, ctypes.POINTER(vp), vp]
+        L.ogs_sort_keep.argtypes = [vp, i, vp, vp, vp, u64, ctypes.POINTER(vp), vp]
+        L.ogs_profile_enable.argtypes = [vp, i]
+        L.ogs_profile_read.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
+                                       ctypes.POINTER(u64)]
+        L.ogs_profile_reset.argtypes = [vp]
+        L.ogs_segmented_sort.argtypes = [vp, i, vp, vp, vp, u64, u64, ctypes.POINTER(vp), vp]
+        L.ogs_sort_host.argtypes = [vp, i, vp, u64, u64]
+        L.ogs_pass_count.argtypes = [vp, i, u64]
+        L.ogs_generate_host.argtypes = [i, i, u64, u64, vp]
+        L.ogs_tune_set.argtypes = [vp, i, cp, i]
+        L.ogs_tune_get.argtypes = [vp, i, cp, ctypes.POINTER(i)]
+        L.ogs_shape_info.argtypes = [i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.ogs_last_error.restype = cp
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise OgsError(f"ogs error {rc}: {lib().ogs_last_error().decode()}")
+    return rc
+
+
+def generate(kind, dist, n, seed=42):
+    """Seeded host input (ogs_generate_host)."""
+    out = np.empty(n, dtype=NP_DTYPE[kind])
+    check(lib().ogs_generate_host(kind, dist, seed, n, ctypes.c_void_p(out.ctypes.data)))
+    return out
+
+
+def shapes(kind, merge):
+    L = lib()
+    b, p, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    cnt = check(L.ogs_shape_info(kind, merge, 0, ctypes.byref(b), ctypes.byref(p), ctypes.byref(w)))
+    res = []
+    for idx in range(cnt):
+        L.ogs_shape_info(kind, merge, idx, ctypes.byref(b), ctypes.byref(p), ctypes.byref(w))
+        res.append((b.value, p.value, w.value))
+    return res
+
+
+class Context:
+    """An ogs_ctx bound to one device."""
+
+    def __init__(self, device=0):
+        self.h = ctypes.c_void_p()
+        check(lib().ogs_ctx_create(ctypes.byref(self.h), device))
+
+    def close(self):
+        if self.h:
+            lib().ogs_ctx_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- raw device-pointer entry points ------------------------------------
+    def sort_ptr(self, kind, data, scratch, n, stream=0):
+        out = ctypes.c_void_p()
+        check(lib().ogs_sort(self.h, kind, ctypes.c_void_p(data), ctypes.c_void_p(scratch), n,
+                             ctypes.byref(out), ctypes.c_void_p(stream)))
+        return out.value
+
+    def sort_keep_ptr(self, kind, src, data, scratch, n, stream=0):
+        out = ctypes.c_void_p()
+        check(lib().ogs_sort_keep(self.h, kind, ctypes.c_void_p(src), ctypes.c_void_p(data),
+                                  ctypes.c_void_p(scratch), n, ctypes.byref(out), ctypes.c_void_p(stream)))
+        return out.value
+
+    def profile(self, on):
+        check(lib().ogs_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        check(lib().ogs_profile_reset(self.h))
+
+    def profile_read(self, kind):
+        """kind 0 block sort, 1 merge partition, 2 merge pass -> (ms, launches, items)."""
+        ms, cnt, items = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().ogs_profile_read(self.h, kind, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(items)))
+        return ms.value, cnt.value, items.value
+
+    def sort_chunks_ptr(self, kind, data, scratch, n, chunk, stream=0):
+        out = ctypes.c_void_p()
+        check(lib().ogs_sort_chunks(self.h, kind, ctypes.c_void_p(data), ctypes.c_void_p(scratch), n, chunk,
+                                    ctypes.byref(out), ctypes.c_void_p(stream)))
+        return out.value
+
+    def segmented_ptr(self, kind, data, scratch, offsets, nseg, n, stream=0):
+        out = ctypes.c_void_p()
+        check(lib().ogs_segmented_sort(self.h, kind, ctypes.c_void_p(data), ctypes.c_void_p(scratch),
+                                       ctypes.c_void_p(offsets), nseg, n, ctypes.byref(out), ctypes.c_void_p(stream)))
+        return out.value
+
+    def sort_host(self, kind, arr, chunk=0):
+        a = np.ascontiguousarray(arr)
+        check(lib().ogs_sort_host(self.h, kind, ctypes.c_void_p(a.ctypes.data), a.size, chunk))
+        return a
+
+    def pass_count(self, kind, n):
+        return check(lib().ogs_pass_count(self.h, kind, n))
+
+    def tune_set(self, kind, param, value):
+        check(lib().ogs_tune_set(self.h, kind, param.encode(), value))
+
+    def tune_get(self, kind, param):
+        v = ctypes.c_int()
+        check(lib().ogs_tune_get(self.h, kind, param.encode(), ctypes.byref(v)))
+        return v.value
+
+    # -- torch-tensor helpers (device memory plumbing) ------------------------
+    def sort_tensor(self, kind, data, scratch, n=None, stream=0):
+        """Sort uint8/int tensors viewed as raw item bytes; returns the tensor
+        (data or scratch) that holds the result."""
+        n = data.numel() * data.element_size() // WIDTH[kind] if n is None else n
+        p = self.sort_ptr(kind, data.data_ptr(), scratch.data_ptr(), n, stream)
+        return data if p == data.data_ptr() else scratch
+
+
+def to_device(arr, device="cuda"):
+    """numpy structured/int array -> torch uint8 device tensor of its bytes."""
+    import torch
+    b = np.ascontiguousarray(arr).view(np.uint8)
+    return torch.from_numpy(b.copy()).to(device)
+
+
+def from_device(t, kind, n):
+    return t[: n * WIDTH[kind]].cpu().numpy().view(NP_DTYPE[kind]).copy()
+
+
+def declared_symbols():
+    """Function names declared in include/ogs.h."""
+    import re
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*(ogs_\w+)\s*\(", txt, re.M)))

@@ -1,0 +1,173 @@
+"""GPU parity: the HIP library (through its C-ABI) against the oracle on the same
+seeded inputs.  Key-only outputs must be bit-exact; KV outputs must have the
+bit-exact key sequence and, per key, the same multiset of values (Origami is
+not stable, Origami/merger.h:84 vs 1002 vs merge_tree.h:40)."""
+import sys
+
+import numpy as np
+import pytest
+
+import ogs
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+KV = (ogs.KV_U64U64, ogs.KV_I64I64)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    c = ogs.Context(0)
+    yield c
+    c.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_sort(ctx, kind, a):
+    import torch
+    n = a.size
+    w = ogs.WIDTH[kind]
+    d = ogs.to_device(a) if n else torch.zeros(16, dtype=torch.uint8, device="cuda")
+    s = torch.empty_like(d)
+    res = ctx.sort_tensor(kind, d, s, n)
+    torch.cuda.synchronize()
+    return ogs.from_device(res, kind, n) if n else a.copy()
+
+
+def assert_same(kind, got, a):
+    ref = oracle.sort(kind, a)
+    if kind in KV:
+        assert np.array_equal(got["key"], ref["key"]), "key sequence differs"
+        # values are a permutation of the input's and each sits under its own key
+        assert np.array_equal(np.sort(got["value"]), np.sort(a["value"]))
+        order_in = np.lexsort((a["value"], a["key"]))
+        order_out = np.lexsort((got["value"], got["key"]))
+        assert np.array_equal(a[order_in].view(np.uint8), got[order_out].view(np.uint8))
+    else:
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+SIZES = [0, 1, 2, 63, 64, 65, 1000, 2047, 2048, 2049, 4095, 4096, 4097, 8191, 8192, 12289,
+         65536, 100003, 1 << 18, (1 << 20) + 17]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_u32_uniform_sizes(ctx, n):
+    say(f"case u32 uniform n={n}")
+    a = ogs.generate(ogs.U32, ogs.UNIFORM, n, seed=42 + n)
+    assert_same(ogs.U32, gpu_sort(ctx, ogs.U32, a), a)
+
+
+def test_u32_config1_2pow20_seed42(ctx):
+    # 2^20 uint32 keys, raw mt19937(42): the reference's single-threaded case
+    say("case u32 2^20 seed 42")
+    a = ogs.generate(ogs.U32, ogs.UNIFORM, 1 << 20, seed=42)
+    got = gpu_sort(ctx, ogs.U32, a)
+    assert np.array_equal(got, np.sort(a))
+    assert_same(ogs.U32, got, a)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.I32, ogs.F32, ogs.U64, ogs.I64, ogs.KV_U64U64, ogs.KV_I64I64])
+@pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.ZIPF, ogs.SORTED, ogs.REVERSE, ogs.DISTINCT16])
+@pytest.mark.parametrize("n", [777, 5000, 300007])
+def test_kinds_and_distributions(ctx, kind, dist, n):
+    say(f"case kind={ogs.KIND_NAMES[kind]} dist={dist} n={n}")
+    a = ogs.generate(kind, ogs.NORMAL if kind == ogs.F32 and dist == ogs.UNIFORM else dist, n, seed=7 + n)
+    assert_same(kind, gpu_sort(ctx, kind, a), a)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.I32, ogs.I64, ogs.KV_I64I64])
+@pytest.mark.parametrize("n", [5, 300, 3001, 70001])
+def test_extreme_keys(ctx, kind, n):
+    # maximum and minimum keys mixed with pads in partial tiles
+    say(f"case extremes kind={ogs.KIND_NAMES[kind]} n={n}")
+    rng = np.random.default_rng(n)
+    a = ogs.generate(kind, ogs.UNIFORM, n, seed=n)
+    raw = a.view(np.uint8).reshape(n, -1)
+    pick = rng.random(n)
+    w = ogs.WIDTH[kind]
+    kw = min(w, 8)
+    raw[pick < 0.2, :kw] = 0xFF
+    raw[(pick >= 0.2) & (pick < 0.3), :kw] = 0x00
+    if kind in (ogs.I32, ogs.I64, ogs.KV_I64I64):
+        raw[(pick >= 0.3) & (pick < 0.4), kw - 1] = 0x80
+        raw[(pick >= 0.3) & (pick < 0.4), : kw - 1] = 0
+        raw[(pick >= 0.4) & (pick < 0.5), kw - 1] = 0x7F
+    assert_same(kind, gpu_sort(ctx, kind, a), a)
+
+
+def test_f32_specials(ctx):
+    say("case f32 specials")
+    vals = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 1e-45, -1e-45, 3.4e38, -3.4e38, 1.0, -1.0],
+                    np.float32)
+    a = np.tile(vals, 1000)
+    np.random.default_rng(1).shuffle(a)
+    assert_same(ogs.F32, gpu_sort(ctx, ogs.F32, a), a)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32])
+def test_chunked_sort(ctx, kind):
+    import torch
+    n, chunk = 300000, 4096 * 3
+    say(f"case chunks kind={ogs.KIND_NAMES[kind]}")
+    a = ogs.generate(kind, ogs.NORMAL if kind == ogs.F32 else ogs.UNIFORM, n, seed=3)
+    d = ogs.to_device(a)
+    s = torch.empty_like(d)
+    p = ctx.sort_chunks_ptr(kind, d.data_ptr(), s.data_ptr(), n, chunk)
+    torch.cuda.synchronize()
+    got = ogs.from_device(d if p == d.data_ptr() else s, kind, n)
+    ref = oracle.sort(kind, a, sort_n=chunk)
+    if kind in KV:
+        assert np.array_equal(got["key"], ref["key"])
+    else:
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.I64, ogs.KV_U64U64, ogs.F32])
+def test_segmented_sort(ctx, kind):
+    import torch
+    say(f"case segmented kind={ogs.KIND_NAMES[kind]}")
+    rng = np.random.default_rng(11)
+    lens = np.floor(2.0 ** (rng.random(600) * 17)).astype(np.int64)
+    lens[:5] = [0, 1, 0, 2, 4096]
+    lens[5:9] = [4097, 8192, 8193, 1 << 17]
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    n = int(off[-1])
+    a = ogs.generate(kind, ogs.NORMAL if kind == ogs.F32 else ogs.ZIPF, n, seed=5)
+    d = ogs.to_device(a)
+    s = torch.empty_like(d)
+    doff = torch.from_numpy(off.view(np.int64).copy()).cuda()
+    p = ctx.segmented_ptr(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+    torch.cuda.synchronize()
+    got = ogs.from_device(d if p == d.data_ptr() else s, kind, n)
+    ref = oracle.segmented_sort(kind, a, off, threads=4)
+    if kind in KV:
+        assert np.array_equal(got["key"], ref["key"])
+        for i in range(lens.size):
+            g = got[off[i]:off[i + 1]]
+            x = a[off[i]:off[i + 1]]
+            assert np.array_equal(np.sort(g["value"]), np.sort(x["value"]))
+    else:
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+def test_host_entry_point(ctx):
+    say("case ogs_sort_host")
+    a = ogs.generate(ogs.I64, ogs.UNIFORM, 123457, seed=9)
+    got = ctx.sort_host(ogs.I64, a.copy())
+    assert np.array_equal(got, np.sort(a))
+
+
+def test_sort_is_repeatable(ctx):
+    # the same input sorted twice gives the same bytes (also for KV ties)
+    say("case repeatability")
+    a = ogs.generate(ogs.KV_U64U64, ogs.DISTINCT16, 200003, seed=1)
+    g1 = gpu_sort(ctx, ogs.KV_U64U64, a)
+    g2 = gpu_sort(ctx, ogs.KV_U64U64, a)
+    assert np.array_equal(g1.view(np.uint8), g2.view(np.uint8))
