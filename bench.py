@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--kind", default="u32", choices=["u32", "f32", "u64", "kv_u64u64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="after timing, run three 1-input-sized device copies (torch copy_) as the "
+                         "known-size reference kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -157,6 +160,11 @@ def main():
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
+    if args.copy_calib:
+        for _ in range(3):
+            d_b.copy_(d_in)
+        torch.cuda.synchronize()
+        line["copy_calib_bytes"] = int(d_in.numel())
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
         import oracle

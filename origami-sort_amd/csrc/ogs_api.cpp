@@ -194,24 +194,18 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
         prof.pending.push_back({PK_BLOCK, ev[0], ev[1], n, true});
     }
     if (!one) {
-        const uint64_t ntiles = (n + ms.tile() - 1) / ms.tile();
-        Workspace *w = workspace(s);
-        int rc = w->reserve((ntiles + 1) * sizeof(uint64_t));
-        if (rc) return rc;
         for (uint64_t R = T; R < n; R *= 2) {
             const bool last = 2 * R >= n;
-            hipEvent_t *evp = nullptr;
             if (prof.on) {
-                for (int k = 0; k < 3; k++) ev[k] = prof.get();
-                evp = ev;
+                ev[0] = prof.get();
+                ev[1] = prof.get();
+                (void)hipEventRecord(ev[0], s);
             }
-            e = ms.run(src, dst, n, R, (uint64_t *)w->dev, last ? ki.codec : 0, s, evp);
+            e = ms.run(src, dst, n, R, last ? ki.codec : 0, s);
             if (e != hipSuccess) return hip_fail(e, "merge pass launch");
             if (prof.on) {
-                // ev[1] closes the partition and opens the merge; the
-                // partition record owns it
-                prof.pending.push_back({PK_PARTITION, ev[0], ev[1], n, true});
-                prof.pending.push_back({PK_MERGE, ev[1], ev[2], n, false});
+                (void)hipEventRecord(ev[1], s);
+                prof.pending.push_back({PK_MERGE, ev[0], ev[1], n, true});
             }
             std::swap(src, dst);
         }
@@ -308,14 +302,12 @@ int Ctx::segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_of
     const size_t o_loff = bytes; bytes = align(bytes + n_l * 8);
     const size_t o_lcnt = bytes; bytes = align(bytes + n_l * 4);
     const size_t o_desc = bytes; bytes = align(bytes + n_d * sizeof(SegTileDesc));
-    size_t max_pass = 0;
-    for (int p = 0; p < max_p; p++) max_pass = std::max<size_t>(max_pass, pass_cnt[p]);
-    const size_t o_part = bytes; bytes = align(bytes + (max_pass + 1) * 8);
+    const size_t o_end = bytes;
     Workspace *w = workspace(s);
     int rc = w->reserve(bytes);
     if (rc) return rc;
     std::vector<uint8_t> &h = w->host;
-    h.resize(o_part);
+    h.resize(o_end);
     if (n_s) {
         memcpy(h.data() + o_soff, s_off.data(), n_s * 8);
         memcpy(h.data() + o_scnt, s_cnt.data(), n_s * 4);
@@ -326,7 +318,7 @@ int Ctx::segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_of
     }
     if (n_d) memcpy(h.data() + o_desc, desc.data(), n_d * sizeof(SegTileDesc));
     // pageable-source copy: returns once the source has been consumed
-    e = hipMemcpyAsync(w->dev, h.data(), o_part, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync(w->dev, h.data(), o_end, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
     uint8_t *D = (uint8_t *)w->dev;
     e = sb.run_seg(data, scratch, data, (const uint64_t *)(D + o_soff), (const uint32_t *)(D + o_scnt), n_s,
@@ -337,7 +329,7 @@ int Ctx::segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_of
     if (e != hipSuccess) return hip_fail(e, "large-segment tile launch");
     for (int p = 0; p < max_p; p++) {
         e = mb.run_seg(data, scratch, D + o_desc + pass_start[p] * sizeof(SegTileDesc), pass_cnt[p], LT << p,
-                       (uint64_t *)(D + o_part), ki.codec, s);
+                       ki.codec, s);
         if (e != hipSuccess) return hip_fail(e, "segmented merge launch");
     }
     return OGS_OK;

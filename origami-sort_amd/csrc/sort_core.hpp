@@ -72,15 +72,17 @@ struct GMem {
     }
 };
 
-// LDS index padding: one spare slot every 32 banks' worth of keys keeps the
-// strided per-lane streams of the merge path on distinct banks.
-template <typename K> __device__ __forceinline__ int lpad(int i) {
-    if constexpr (sizeof(K) == 4) return i + (i >> 5);
-    else return i + (i >> 4);
+// LDS layout: XOR swizzle of the 16-byte chunk index within each 128-byte
+// row by the row number.  A lane writing its IPT consecutive items as 16-byte
+// stores then hits distinct banks from its neighbours (plain layout: lanes 0
+// and 2 collide), and lane-consecutive accesses stay a permutation within a
+// row.  Items of width T; the swizzle never moves an item out of its chunk.
+template <typename T> __device__ __forceinline__ int lsw(int i) {
+    if constexpr (sizeof(T) == 4) return i ^ (((i >> 5) & 7) << 2);
+    else return i ^ (((i >> 4) & 7) << 1);
 }
-template <typename K> constexpr int lpad_size(int n) {
-    return sizeof(K) == 4 ? n + (n >> 5) + 2 : n + (n >> 4) + 2;
-}
+// items per 16-byte chunk
+template <typename T> constexpr int chunk_items() { return 16 / (int)sizeof(T); }
 
 // ----------------------------------------------------------------------------
 // Compare-exchange (SWAP / SWAPKV2, commons.h:111-149): a <- smaller, b <- larger.
@@ -216,8 +218,8 @@ __device__ __forceinline__ int lds_corank(const K *sk, int a_off, int la, int b_
     int hi = diag < la ? diag : la;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        K a = sk[lpad<K>(a_off + mid)];
-        K b = sk[lpad<K>(b_off + diag - 1 - mid)];
+        K a = sk[lsw<K>(a_off + mid)];
+        K b = sk[lsw<K>(b_off + diag - 1 - mid)];
         if (!(b < a)) lo = mid + 1;
         else hi = mid;
     }
@@ -233,11 +235,11 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
                                                  uint64_t (&v)[IPT]) {
     int ia = a_off + ai, ib = b_off + bi;
     const int ea = a_off + la, eb = b_off + lb;
-    K ka = sk[lpad<K>(min(ia, limit))], kb = sk[lpad<K>(min(ib, limit))];
+    K ka = sk[lsw<K>(min(ia, limit))], kb = sk[lsw<K>(min(ib, limit))];
     uint64_t va = 0, vb = 0;
     if constexpr (HV) {
-        va = sv[lpad<K>(min(ia, limit))];
-        vb = sv[lpad<K>(min(ib, limit))];
+        va = sv[lsw<uint64_t>(min(ia, limit))];
+        vb = sv[lsw<uint64_t>(min(ib, limit))];
     }
 #pragma unroll
     for (int i = 0; i < IPT; i++) {
@@ -248,11 +250,11 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
             ia += takeA;
             ib += !takeA;
             int nx = min(takeA ? ia : ib, limit);
-            K nk = sk[lpad<K>(nx)];
+            K nk = sk[lsw<K>(nx)];
             ka = takeA ? nk : ka;
             kb = takeA ? kb : nk;
             if constexpr (HV) {
-                uint64_t nv = sv[lpad<K>(nx)];
+                uint64_t nv = sv[lsw<uint64_t>(nx)];
                 va = takeA ? nv : va;
                 vb = takeA ? vb : nv;
             }
@@ -260,14 +262,121 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
     }
 }
 
-// Store IPT consecutive registers to LDS at item index base (padded layout).
+// Store IPT consecutive registers to LDS at item index base (a multiple of
+// the chunk size) as 16-byte stores.
 template <int IPT, bool HV, typename K>
 __device__ __forceinline__ void lds_put(K *sk, uint64_t *sv, int base, const K (&r)[IPT], const uint64_t (&v)[IPT]) {
+    constexpr int E = chunk_items<K>();
+    static_assert(IPT % E == 0, "IPT must fill whole 16-byte chunks");
 #pragma unroll
-    for (int i = 0; i < IPT; i++) {
-        sk[lpad<K>(base + i)] = r[i];
-        if constexpr (HV) sv[lpad<K>(base + i)] = v[i];
+    for (int c = 0; c < IPT; c += E) {
+        if constexpr (E == 4) {
+            uint4 q = {(uint32_t)r[c], (uint32_t)r[c + 1], (uint32_t)r[c + 2], (uint32_t)r[c + 3]};
+            *(uint4 *)&sk[lsw<K>(base + c)] = q;
+        } else {
+            ulonglong2 q;
+            q.x = (unsigned long long)r[c];
+            q.y = (unsigned long long)r[c + 1];
+            *(ulonglong2 *)&sk[lsw<K>(base + c)] = q;
+        }
     }
+    if constexpr (HV) {
+#pragma unroll
+        for (int c = 0; c < IPT; c += 2) {
+            ulonglong2 q;
+            q.x = v[c];
+            q.y = v[c + 1];
+            *(ulonglong2 *)&sv[lsw<uint64_t>(base + c)] = q;
+        }
+    }
+}
+
+// Writes items [0, cnt) of an LDS tile to out[dst ..] with lane-consecutive,
+// fully coalesced stores (16-byte stores when the destination is aligned),
+// applying the output codec.
+template <int BLOCK, int TILE, bool HV, typename K>
+__device__ __forceinline__ void lds_store_tile(const K *sk, const uint64_t *sv, void *out, uint64_t dst, int cnt,
+                                               int cout) {
+    const int tid = threadIdx.x;
+    if constexpr (HV) {
+        ulonglong2 *o = (ulonglong2 *)out + dst;
+#pragma unroll 4
+        for (int e = tid; e < cnt; e += BLOCK) {
+            ulonglong2 q;
+            q.x = (unsigned long long)dec_key(sk[lsw<K>(e)], cout);
+            q.y = sv[lsw<uint64_t>(e)];
+            o[e] = q;
+        }
+    } else {
+        constexpr int E = chunk_items<K>();
+        K *o = (K *)out + dst;
+        const bool al16 = ((uintptr_t)o & 15) == 0;
+        if (al16 && cnt == TILE) {
+#pragma unroll
+            for (int q = tid; q < TILE / E; q += BLOCK) {
+                const int e = q * E;
+                if constexpr (E == 4) {
+                    uint4 x = *(const uint4 *)&sk[lsw<K>(e)];
+                    x.x = (uint32_t)dec_key((uint32_t)x.x, cout);
+                    x.y = (uint32_t)dec_key((uint32_t)x.y, cout);
+                    x.z = (uint32_t)dec_key((uint32_t)x.z, cout);
+                    x.w = (uint32_t)dec_key((uint32_t)x.w, cout);
+                    *(uint4 *)&o[e] = x;
+                } else {
+                    ulonglong2 x = *(const ulonglong2 *)&sk[lsw<K>(e)];
+                    x.x = dec_key((uint64_t)x.x, cout);
+                    x.y = dec_key((uint64_t)x.y, cout);
+                    *(ulonglong2 *)&o[e] = x;
+                }
+            }
+        } else {
+#pragma unroll 4
+            for (int e = tid; e < cnt; e += BLOCK) o[e] = dec_key(sk[lsw<K>(e)], cout);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
+// Wave-cooperative co-rank in global memory (merge path, Partition.h:117-166
+// specialised to two runs): returns the number of A items among the first
+// `diag` outputs of merge(A, B), ties to A.  Each step evaluates 64 evenly
+// spaced candidates at once and keeps the bracket the ballot points to, so a
+// run of 2^27 items resolves in five dependent steps.  Must be called by all
+// 64 lanes of a wave with uniform arguments.
+// ----------------------------------------------------------------------------
+template <typename K, bool HV>
+__device__ __forceinline__ uint64_t wave_corank(const void *in, uint64_t a_base, uint64_t la, uint64_t b_base,
+                                                uint64_t lb, uint64_t diag, int lane) {
+    uint64_t lo = diag > lb ? diag - lb : 0;
+    uint64_t hi = diag < la ? diag : la;
+    while (hi > lo) {
+        const uint64_t span = hi - lo;
+        if (span <= 64) {
+            bool p = false;
+            if ((uint64_t)lane < span) {
+                const uint64_t m = lo + lane;
+                const K a = GMem<K, HV>::load_key(in, a_base + m);
+                const K b = GMem<K, HV>::load_key(in, b_base + diag - 1 - m);
+                p = b < a;
+            }
+            const uint64_t mask = __ballot(p);
+            return mask ? lo + (uint64_t)(__ffsll((unsigned long long)mask) - 1) : hi;
+        }
+        const uint64_t m = lo + (span * (uint64_t)(lane + 1)) / 65;
+        const K a = GMem<K, HV>::load_key(in, a_base + m);
+        const K b = GMem<K, HV>::load_key(in, b_base + diag - 1 - m);
+        const uint64_t mask = __ballot(b < a);
+        if (!mask) {
+            lo = lo + (span * 64) / 65 + 1;
+        } else {
+            const int f = __ffsll((unsigned long long)mask) - 1;
+            const uint64_t mf = lo + (span * (uint64_t)(f + 1)) / 65;
+            const uint64_t nlo = f ? lo + (span * (uint64_t)f) / 65 + 1 : lo;
+            hi = mf;
+            lo = nlo;
+        }
+    }
+    return lo;
 }
 
 }  // namespace ogs

@@ -2,10 +2,10 @@
 //
 //   k_block_sort      phase 1-2 of the reference (phase2_sort, sorter.h:3831-3849):
 //                     one workgroup sorts a tile in registers + LDS.
-//   k_merge_partition co-rank of every merge tile (Partition_Ptrs, Partition.h:117-166,
-//                     specialised to two runs).
 //   k_merge_pass      one 2-way pass of merge_series (merger.h:4079-4228) as a
-//                     merge-path merge staged in LDS.
+//                     merge-path merge staged in LDS; each workgroup finds its own
+//                     co-ranks with a wave-wide search (Partition_Ptrs,
+//                     Partition.h:117-166, specialised to two runs).
 //   k_seg_*           the same kernels driven by per-tile descriptors, for batches
 //                     of independent segments (uniform chunks in the reference:
 //                     sort_single_thread's sort_n, sorter.h:3857-3868).
@@ -22,8 +22,8 @@ namespace ogs {
 // largest there is, so they belong at the end in any order).
 // ----------------------------------------------------------------------------
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-__device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t base, int cnt,
-                                          int cin, int cout, K *sk, uint64_t *sv, int *smisc) {
+__device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t base, int cnt, int cin, int cout,
+                                          K *sk, uint64_t *sv, int *smisc) {
     constexpr int TILE = BLOCK * IPT;
     constexpr K KMAX = KeyMax<K>::v;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -82,23 +82,15 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     __syncthreads();
     lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < IPT; i++) {
-        const int e = i * BLOCK + tid;
-        if (e < lim) {
-            uint64_t vv = 0;
-            if constexpr (HV) vv = sv[lpad<K>(e)];
-            GMem<K, HV>::store(out, base + e, dec_key(sk[lpad<K>(e)], cout), vv);
-        }
-    }
+    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, base, lim, cout);
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
                                                       int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ K sk[lpad_size<K>(TILE)];
-    __shared__ uint64_t sv[HV ? lpad_size<K>(TILE) : 1];
+    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ int smisc[2];
     const uint64_t base = (uint64_t)blockIdx.x * TILE;
     const int cnt = (int)(n - base < (uint64_t)TILE ? n - base : (uint64_t)TILE);
@@ -116,8 +108,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const void *in, void *o
                                                          const uint64_t *__restrict__ tile_off,
                                                          const uint32_t *__restrict__ tile_cnt, int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ K sk[lpad_size<K>(TILE)];
-    __shared__ uint64_t sv[HV ? lpad_size<K>(TILE) : 1];
+    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ int smisc[2];
     const uint64_t base = tile_off[blockIdx.x];
     const uint32_t c = tile_cnt[blockIdx.x];
@@ -129,7 +121,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const void *in, void *o
 // ----------------------------------------------------------------------------
 // 2-way merge pass.  Runs of length R at multiples of 2R (relative to a segment
 // base) are merged pairwise; output tile j of a segment covers items
-// [j*TILE, j*TILE+TILE).  part[j] is the co-rank of the tile start.
+// [j*TILE, j*TILE+TILE).
 // ----------------------------------------------------------------------------
 struct PairGeom {
     uint64_t pb, la, lb;  // pair base (relative), run lengths
@@ -143,145 +135,91 @@ __device__ __forceinline__ PairGeom pair_geom(uint64_t o, uint64_t n, uint64_t R
     return g;
 }
 
-template <typename K, bool HV>
-__device__ __forceinline__ uint64_t global_corank(const void *in, uint64_t seg, uint64_t a_base, uint64_t la,
-                                                  uint64_t b_base, uint64_t lb, uint64_t diag) {
-    uint64_t lo = diag > lb ? diag - lb : 0;
-    uint64_t hi = diag < la ? diag : la;
-    while (lo < hi) {
-        const uint64_t mid = (lo + hi) >> 1;
-        const K a = GMem<K, HV>::load_key(in, seg + a_base + mid);
-        const K b = GMem<K, HV>::load_key(in, seg + b_base + diag - 1 - mid);
-        if (!(b < a)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-template <typename K, bool HV>
-__global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint64_t R, uint32_t tile,
-                                  uint64_t ntiles, uint64_t *__restrict__ part) {
-    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ntiles) return;
-    const uint64_t o = j * tile;
-    const PairGeom g = pair_geom(o, n, R);
-    part[j] = global_corank<K, HV>(in, 0, g.pb, g.la, g.pb + g.la, g.lb, o - g.pb);
-}
-
-// Merges one output tile: loads the tile's A and B slices into LDS (one
-// coalesced stream), runs the per-thread merge path, stores IPT consecutive
-// items per thread.
+// Merges one output tile: waves 0 and 1 find the co-ranks of the tile's start
+// and end, the tile's A and B slices are loaded into LDS as one coalesced
+// stream, each thread merges IPT outputs along its merge path, and the tile is
+// written back through LDS with lane-consecutive stores.
 template <typename K, bool HV, int BLOCK, int IPT>
 __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__restrict__ out, uint64_t seg,
-                                           uint64_t n, uint64_t R, uint64_t o0, uint64_t a0, uint64_t a1_next,
-                                           int cout, K *sk, uint64_t *sv) {
+                                           uint64_t n, uint64_t R, uint64_t o0, int cout, K *sk, uint64_t *sv,
+                                           uint64_t *sco) {
+    static_assert(BLOCK >= 128, "the co-rank searches need two waves");
     constexpr int TILE = BLOCK * IPT;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int cnt = (int)(n - o0 < (uint64_t)TILE ? n - o0 : (uint64_t)TILE);
     const PairGeom g = pair_geom(o0, n, R);
     const uint64_t d0 = o0 - g.pb, d1 = d0 + cnt;
-    const uint64_t a1 = (d1 == g.la + g.lb) ? g.la : a1_next;
+    const uint64_t abase = seg + g.pb, bbase = seg + g.pb + g.la;
+    if (wave == 0) {
+        const uint64_t a = wave_corank<K, HV>(in, abase, g.la, bbase, g.lb, d0, lane);
+        if (lane == 0) sco[0] = a;
+    } else if (wave == 1) {
+        const uint64_t a = (d1 == g.la + g.lb) ? g.la : wave_corank<K, HV>(in, abase, g.la, bbase, g.lb, d1, lane);
+        if (lane == 0) sco[1] = a;
+    }
+    __syncthreads();
+    const uint64_t a0 = sco[0], a1 = sco[1];
     const int la = (int)(a1 - a0);
     const int lb = cnt - la;
-    const uint64_t A = seg + g.pb + a0;
-    const uint64_t B = seg + g.pb + g.la + (d0 - a0);
+    const uint64_t A = abase + a0;
+    const uint64_t B = bbase + (d0 - a0);
 #pragma unroll 4
     for (int e = tid; e < cnt; e += BLOCK) {
         const uint64_t src = e < la ? A + e : B + (e - la);
         K k;
         uint64_t vv = 0;
         GMem<K, HV>::load(in, src, k, vv);
-        sk[lpad<K>(e)] = k;
-        if constexpr (HV) sv[lpad<K>(e)] = vv;
+        sk[lsw<K>(e)] = k;
+        if constexpr (HV) sv[lsw<uint64_t>(e)] = vv;
     }
     __syncthreads();
     const int d = tid * IPT;
+    K r[IPT];
+    uint64_t v[IPT];
     if (d < cnt) {
         const int c = cnt - d < IPT ? cnt - d : IPT;
         const int a = lds_corank<K>(sk, 0, la, la, lb, d);
-        K r[IPT];
-        uint64_t v[IPT];
         lds_merge_serial<IPT, HV>(sk, sv, 0, la, la, lb, a, d - a, c, cnt - 1, r, v);
-        const uint64_t ob = seg + o0 + d;
-        const bool al16 = ((uintptr_t)((K *)out + ob) & 15) == 0;
-        if (c == IPT && al16 && !HV && sizeof(K) == 4 && (IPT % 4) == 0) {
-#pragma unroll
-            for (int i = 0; i < IPT; i += 4) {
-                uint4 q;
-                q.x = (uint32_t)dec_key(r[i + 0], cout);
-                q.y = (uint32_t)dec_key(r[i + 1], cout);
-                q.z = (uint32_t)dec_key(r[i + 2], cout);
-                q.w = (uint32_t)dec_key(r[i + 3], cout);
-                *(uint4 *)((K *)out + ob + i) = q;
-            }
-        } else if (c == IPT && al16 && !HV && sizeof(K) == 8 && (IPT % 2) == 0) {
-#pragma unroll
-            for (int i = 0; i < IPT; i += 2) {
-                ulonglong2 q;
-                q.x = (unsigned long long)dec_key(r[i + 0], cout);
-                q.y = (unsigned long long)dec_key(r[i + 1], cout);
-                *(ulonglong2 *)((K *)out + ob + i) = q;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < IPT; i++)
-                if (i < c) GMem<K, HV>::store(out, ob + i, dec_key(r[i], cout), v[i]);
-        }
     }
+    __syncthreads();
+    if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
+    __syncthreads();
+    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, seg + o0, cnt, cout);
 }
 
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
-                                                      uint64_t R, const uint64_t *__restrict__ part, int cout) {
+                                                      uint64_t R, int cout) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ K sk[lpad_size<K>(TILE)];
-    __shared__ uint64_t sv[HV ? lpad_size<K>(TILE) : 1];
-    const uint64_t j = blockIdx.x;
-    const uint64_t o0 = j * TILE;
-    const uint64_t a1n = (o0 + TILE < n) ? part[j + 1] : 0;
-    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, o0, part[j], a1n, cout, sk, sv);
+    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    __shared__ uint64_t sco[2];
+    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco);
 }
 
-// Segmented forms: descriptor t = {segment base, segment length, tile start
+// Segmented form: descriptor t = {segment base, segment length, tile start
 // within the segment}; one entry per output tile of every segment that still
 // has runs of length R to merge.  Bit 63 of o0 is the direction: 0 reads buf0
 // and writes buf1, 1 the reverse (segments with different pass counts sit on
-// different buffers in the same pass); bit 62 marks the segment's last pass.  part[t] is the co-rank of the tile
-// start; the end co-rank comes from descriptor t+1, which the host lays out
-// consecutively within a segment.
+// different buffers in the same pass); bit 62 marks the segment's last pass.
 struct SegTile {
     uint64_t seg, len, o0;
 };
 static constexpr uint64_t kDirBit = 1ull << 63;
 static constexpr uint64_t kLastBit = 1ull << 62;
 
-template <typename K, bool HV>
-__global__ void k_seg_merge_partition(const void *__restrict__ buf0, const void *__restrict__ buf1,
-                                      const SegTile *__restrict__ tiles, uint64_t ntiles, uint64_t R,
-                                      uint64_t *__restrict__ part) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    const SegTile d = tiles[t];
-    const void *in = (d.o0 & kDirBit) ? buf1 : buf0;
-    const uint64_t o0 = d.o0 & ~(kDirBit | kLastBit);
-    const PairGeom g = pair_geom(o0, d.len, R);
-    part[t] = global_corank<K, HV>(in, d.seg, g.pb, g.la, g.pb + g.la, g.lb, o0 - g.pb);
-}
-
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(void *buf0, void *buf1, const SegTile *__restrict__ tiles,
-                                                          uint64_t R, const uint64_t *__restrict__ part, int cout) {
+                                                          uint64_t R, int cout) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ K sk[lpad_size<K>(TILE)];
-    __shared__ uint64_t sv[HV ? lpad_size<K>(TILE) : 1];
-    const uint64_t t = blockIdx.x;
-    const SegTile d = tiles[t];
+    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    __shared__ uint64_t sco[2];
+    const SegTile d = tiles[blockIdx.x];
     const bool dir = (d.o0 & kDirBit) != 0;
     const int co = (d.o0 & kLastBit) ? cout : 0;
     const uint64_t o0 = d.o0 & ~(kDirBit | kLastBit);
-    const uint64_t a1n = (o0 + TILE < d.len) ? part[t + 1] : 0;
-    merge_tile<K, HV, BLOCK, IPT>(dir ? buf1 : buf0, dir ? buf0 : buf1, d.seg, d.len, R, o0, part[t], a1n, co, sk,
-                                  sv);
+    merge_tile<K, HV, BLOCK, IPT>(dir ? buf1 : buf0, dir ? buf0 : buf1, d.seg, d.len, R, o0, co, sk, sv, sco);
 }
 
 // ----------------------------------------------------------------------------
@@ -308,29 +246,21 @@ static hipError_t launch_seg_block(const void *in, void *out_even, void *out_odd
 }
 
 template <typename K, bool HV, int BLOCK, int IPT>
-static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, uint64_t *part, int cout,
-                               hipStream_t s, hipEvent_t *ev) {
+static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, int cout, hipStream_t s) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     const uint64_t ntiles = (n + TILE - 1) / TILE;
     if (ntiles == 0) return hipSuccess;
-    if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL((k_merge_partition<K, HV>), dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, s, in, n, R,
-                       (uint32_t)TILE, ntiles, part);
-    if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL((k_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n, R,
-                       (const uint64_t *)part, cout);
-    if (ev) (void)hipEventRecord(ev[2], s);
+                       cout);
     return hipGetLastError();
 }
 
 template <typename K, bool HV, int BLOCK, int IPT>
-static hipError_t launch_seg_merge(void *buf0, void *buf1, const void *tiles, uint64_t ntiles, uint64_t R,
-                                   uint64_t *part, int cout, hipStream_t s) {
+static hipError_t launch_seg_merge(void *buf0, void *buf1, const void *tiles, uint64_t ntiles, uint64_t R, int cout,
+                                   hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_seg_merge_partition<K, HV>), dim3((unsigned)((ntiles + 255) / 256)), dim3(256), 0, s,
-                       (const void *)buf0, (const void *)buf1, (const SegTile *)tiles, ntiles, R, part);
     hipLaunchKernelGGL((k_seg_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, buf0, buf1,
-                       (const SegTile *)tiles, R, (const uint64_t *)part, cout);
+                       (const SegTile *)tiles, R, cout);
     return hipGetLastError();
 }
 
@@ -338,12 +268,14 @@ static hipError_t launch_seg_merge(void *buf0, void *buf1, const void *tiles, ui
 // (BLOCK, IPT, WLEV), merge shapes (BLOCK, IPT).  The first entry of each list
 // is the default; the tuning table may pick any other.
 #define OGS_BLOCK_SHAPES_32(X) X(256, 16, 1) X(256, 16, 0) X(256, 16, 2) X(256, 16, 3) X(256, 32, 1) \
-    X(512, 16, 1) X(128, 16, 1) X(256, 8, 1) X(64, 16, 1)
-#define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8)
-#define OGS_BLOCK_SHAPES_64(X) X(256, 16, 1) X(256, 8, 1) X(128, 16, 1) X(512, 8, 1) X(64, 16, 1)
-#define OGS_MERGE_SHAPES_64(X) X(256, 16) X(256, 8) X(512, 8) X(128, 16)
-#define OGS_BLOCK_SHAPES_KV(X) X(256, 8, 1) X(128, 8, 1) X(256, 4, 1) X(512, 4, 1) X(64, 8, 1)
-#define OGS_MERGE_SHAPES_KV(X) X(256, 8) X(128, 8) X(256, 4) X(512, 4)
+    X(512, 16, 1) X(512, 32, 1) X(1024, 16, 1) X(1024, 32, 1) X(128, 16, 1) X(256, 8, 1) X(64, 16, 1) X(64, 8, 1)
+#define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8) X(1024, 16)
+#define OGS_BLOCK_SHAPES_64(X) X(256, 16, 1) X(256, 8, 1) X(128, 16, 1) X(512, 8, 1) X(512, 16, 1) \
+    X(1024, 16, 1) X(64, 16, 1) X(64, 8, 1)
+#define OGS_MERGE_SHAPES_64(X) X(256, 16) X(256, 8) X(512, 8) X(128, 16) X(512, 16)
+#define OGS_BLOCK_SHAPES_KV(X) X(256, 8, 1) X(128, 8, 1) X(256, 4, 1) X(512, 4, 1) X(512, 8, 1) X(64, 8, 1) \
+    X(64, 4, 1)
+#define OGS_MERGE_SHAPES_KV(X) X(256, 8) X(128, 8) X(256, 4) X(512, 4) X(512, 8)
 
 #define OGS_BX(K, HV, B, I, W) {B, I, W, &launch_block<K, HV, B, I, W>, &launch_seg_block<K, HV, B, I, W>},
 #define OGS_MX(K, HV, B, I) {B, I, &launch_merge<K, HV, B, I>, &launch_seg_merge<K, HV, B, I>},

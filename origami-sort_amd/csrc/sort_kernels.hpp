@@ -25,12 +25,9 @@ static constexpr uint32_t kTileFinalBit = 1u << 30;
 typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s);
 typedef hipError_t (*SegBlockLaunch)(const void *in, void *out_even, void *out_odd, const uint64_t *tile_off,
                                      const uint32_t *tile_cnt, uint64_t ntiles, int cin, int cout, hipStream_t s);
-// ev (nullable) = 3 events recorded before the partition kernel, between it
-// and the merge kernel, and after the merge kernel.
-typedef hipError_t (*MergeLaunch)(const void *in, void *out, uint64_t n, uint64_t R, uint64_t *part, int cout,
-                                  hipStream_t s, hipEvent_t *ev);
-typedef hipError_t (*SegMergeLaunch)(void *buf0, void *buf1, const void *tiles, uint64_t ntiles, uint64_t R,
-                                     uint64_t *part, int cout, hipStream_t s);
+typedef hipError_t (*MergeLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, hipStream_t s);
+typedef hipError_t (*SegMergeLaunch)(void *buf0, void *buf1, const void *tiles, uint64_t ntiles, uint64_t R, int cout,
+                                     hipStream_t s);
 
 struct BlockShapeFns {
     int block, ipt, wlev;

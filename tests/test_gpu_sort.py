@@ -171,3 +171,29 @@ def test_sort_is_repeatable(ctx):
     g1 = gpu_sort(ctx, ogs.KV_U64U64, a)
     g2 = gpu_sort(ctx, ogs.KV_U64U64, a)
     assert np.array_equal(g1.view(np.uint8), g2.view(np.uint8))
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64])
+def test_every_compiled_shape(ctx, kind):
+    # every block-sort and merge shape the tuning table may select is parity-checked
+    bshapes = ogs.shapes(kind, 0)
+    mshapes = ogs.shapes(kind, 1)
+    try:
+        for bi, (b, ipt, w) in enumerate(bshapes):
+            ctx.tune_set(kind, "plain_block", bi)
+            mi = bi % len(mshapes)
+            ctx.tune_set(kind, "plain_merge", mi)
+            for n in (b * ipt - 3, 5 * b * ipt + 77):
+                say(f"case shape kind={ogs.KIND_NAMES[kind]} block={bshapes[bi]} merge={mshapes[mi]} n={n}")
+                a = ogs.generate(kind, ogs.UNIFORM if n % 2 else ogs.DISTINCT16, n, seed=n)
+                assert_same(kind, gpu_sort(ctx, kind, a), a)
+        for mi, (b, ipt, w) in enumerate(mshapes):
+            ctx.tune_set(kind, "plain_block", 0)
+            ctx.tune_set(kind, "plain_merge", mi)
+            n = 200003
+            say(f"case merge shape kind={ogs.KIND_NAMES[kind]} merge={mshapes[mi]} n={n}")
+            a = ogs.generate(kind, ogs.ZIPF, n, seed=mi)
+            assert_same(kind, gpu_sort(ctx, kind, a), a)
+    finally:
+        ctx.tune_set(kind, "plain_block", 0)
+        ctx.tune_set(kind, "plain_merge", 0)
