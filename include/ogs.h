@@ -103,6 +103,20 @@ int ogs_segmented_sort(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch
                        const uint64_t *d_offsets, uint64_t nseg, uint64_t n,
                        void **d_sorted, void *stream);
 
+/* Batched sort of independent arrays in one launch sequence: item i sorts
+ * items[i].n items from items[i].in into items[i].out, using items[i].scratch
+ * (n items) as the second buffer.  scratch may be NULL when `in` may be
+ * clobbered (then `in` is the second buffer); in may equal out.  Arrays are
+ * binned by length into the tile table's size classes (see ogs_tune_set).
+ * The serving loop (ogs_server_*) submits its coalesced requests this way. */
+typedef struct {
+    const void *in;
+    void *out;
+    void *scratch;
+    uint64_t n;
+} ogs_batch_item;
+int ogs_sort_batch(ogs_ctx *ctx, ogs_key_t kt, const ogs_batch_item *items, uint64_t count, void *stream);
+
 /* Host-memory convenience: copies h_data (n items) to the device, sorts, and
  * copies the result back into h_data; synchronous.  This is what
  * origami_gpu_compat.h calls for the reference's host-pointer signatures. */
@@ -121,11 +135,21 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
 
 /* Tile table (the run-time counterpart of the tuned constants in
  * Origami/config.h:45-264).  param: "plain_block", "plain_merge",
- * "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge"; block
- * and merge parameters take an index into ogs_shape_info's lists,
- * seg_small_max an item count (0 = the small shape's tile). */
+ * "seg_tiny_block", "seg_tiny_max", "seg_small_block", "seg_small_max",
+ * "seg_large_block", "seg_merge".  Block and merge parameters take an index
+ * into ogs_shape_info's lists; *_max take an item count.  Segments of up to
+ * seg_tiny_max items (0 = no tiny class) get one workgroup of the tiny shape
+ * each, segments up to seg_small_max (0 = the small shape's tile) one
+ * workgroup of the small shape, longer ones are cut into tiles of the large
+ * shape and merged by passes of the seg_merge shape. */
 int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value);
 int ogs_tune_get(ogs_ctx *ctx, ogs_key_t kt, const char *param, int *value);
+
+/* Applies a tile-table file: lines "kind param value" (kind u32, i32, f32,
+ * u64, i64, kv_u64u64 or kv_i64i64), '#' starts a comment.  ogs_ctx_create
+ * applies $OGS_TILES, or else ogs_tiles.txt next to libogs.so, when present;
+ * the autotuner (origami-sort_amd/autotune.py) writes that file. */
+int ogs_tune_load(ogs_ctx *ctx, const char *path);
 
 /* Compiled kernel shapes of a key type: merge = 0 lists block-sort shapes
  * (workgroup size, items per lane, in-wave bitonic levels), merge = 1 merge

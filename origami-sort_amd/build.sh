@@ -14,5 +14,9 @@ for src in csrc/sort_kernels.hip csrc/ogs_api.cpp csrc/ogs_host.cpp; do
     pids+=($!)
 done
 for p in "${pids[@]}"; do wait "$p"; done
-$HIPCC --offload-arch=gfx950 -shared -o libogs.so.tmp _build/*.o -lpthread
+$HIPCC --offload-arch=gfx950 -shared -o libogs.so.tmp _build/*.o -lpthread -ldl
 mv -f libogs.so.tmp libogs.so
+# drop-in check program for include/origami_gpu_compat.h (run by tests/test_gpu_compat.py)
+g++ -O2 -std=c++17 -I ../include ../tests/compat_bench_sort.cpp -o compat_bench_sort.tmp \
+    -L . -logs -ldl -Wl,-rpath,'$ORIGIN' -L /opt/rocm/lib -lamdhip64
+mv -f compat_bench_sort.tmp compat_bench_sort

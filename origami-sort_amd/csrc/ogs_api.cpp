@@ -62,6 +62,8 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_SEG_SMALL_MAX] = 0;  // 0: the small class is the tile of its shape
     t.v[TP_SEG_LARGE_BLOCK] = 0;
     t.v[TP_SEG_MERGE] = 0;
+    t.v[TP_SEG_TINY_BLOCK] = 0;
+    t.v[TP_SEG_TINY_MAX] = 0;
     return t;
 }
 
@@ -90,8 +92,11 @@ int Workspace::reserve(size_t bytes) {
     return OGS_OK;
 }
 
+Workspace::Workspace() { (void)hipEventCreateWithFlags(&staged, hipEventDisableTiming); }
+
 Workspace::~Workspace() {
     if (dev) (void)hipFree(dev);
+    if (staged) (void)hipEventDestroy(staged);
 }
 
 const BlockShapeFns &Ctx::block_shape(KeyClass kc, int param) {
@@ -224,115 +229,135 @@ int Ctx::pass_count(ogs_key_t kt, uint64_t n) {
     return p;
 }
 
-// Segmented sort with host-visible offsets.  Every segment ends in `scratch`.
-//  - segments of 1..small_max items: one tile-sort workgroup each (small shape)
-//  - longer segments: TILE-sized pieces block-sorted (large shape), then
-//    merge passes; a segment needing an odd number of passes block-sorts in
-//    place so that its last pass lands in scratch.
+// Batched sort of independent segments; segment i is sorted from segs[i].in
+// into segs[i].fin, using segs[i].other as its second buffer (`in` may alias
+// either; fin and other must differ).  Size classes:
+//  - up to tiny_max items: one workgroup of the tiny shape per segment
+//  - up to small_max items: one workgroup of the small shape per segment
+//  - longer: TILE-sized pieces block-sorted with the large shape, then merge
+//    passes; the pieces start on the buffer that makes the segment's last
+//    pass land in `fin` (a segment with P passes starts in fin when P is even).
+// All segments of a class share one launch; all segments' p-th merge pass
+// share one launch.
+int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const BlockShapeFns &tb = block_shape(ki.kc, TP_SEG_TINY_BLOCK);
+    const BlockShapeFns &sb = block_shape(ki.kc, TP_SEG_SMALL_BLOCK);
+    const BlockShapeFns &lb = block_shape(ki.kc, TP_SEG_LARGE_BLOCK);
+    const MergeShapeFns &mb = merge_shape(ki.kc, TP_SEG_MERGE, lb.tile());
+    uint64_t small_max = (uint64_t)tune[ki.kc].v[TP_SEG_SMALL_MAX];
+    if (small_max == 0 || small_max > sb.tile()) small_max = sb.tile();
+    uint64_t tiny_max = (uint64_t)tune[ki.kc].v[TP_SEG_TINY_MAX];
+    if (tiny_max > tb.tile()) tiny_max = tb.tile();
+    if (tiny_max > small_max) tiny_max = small_max;
+    const uint64_t LT = lb.tile(), MT = mb.tile();
+    const uint64_t w = (uint64_t)ki.width;
+    auto at = [w](const void *p, uint64_t i) { return (void *)((char *)p + i * w); };
+
+    std::vector<TileDescH> td[3];  // tiny, small, large
+    struct Large { size_t i; int passes; };
+    std::vector<Large> large;
+    for (size_t i = 0; i < nseg; i++) {
+        const SegSpec &g = segs[i];
+        if (g.len == 0) continue;
+        if (!g.in || !g.fin || !g.other || g.fin == g.other)
+            return fail(OGS_ERR_ARG, "segment %zu: NULL buffer or fin == other", i);
+        if (g.len <= tiny_max) {
+            td[0].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
+        } else if (g.len <= small_max) {
+            td[1].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
+        } else {
+            int p = 0;
+            for (uint64_t R = LT; R < g.len; R *= 2) p++;
+            large.push_back({i, p});
+            void *dst = (p & 1) ? g.other : g.fin;
+            const uint32_t fin = p == 0 ? kTileFinalBit : 0u;
+            for (uint64_t o = 0; o < g.len; o += LT)
+                td[2].push_back({at(g.in, o), at(dst, o), (uint32_t)std::min<uint64_t>(LT, g.len - o) | fin, 0});
+        }
+    }
+    int max_p = 0;
+    for (auto &L : large) max_p = std::max(max_p, L.passes);
+    std::vector<MergeDescH> desc;
+    std::vector<uint64_t> pass_start(max_p + 1, 0), pass_cnt(max_p + 1, 0);
+    for (int p = 0; p < max_p; p++) {
+        pass_start[p] = desc.size();
+        for (auto &L : large) {
+            if (L.passes <= p) continue;
+            const SegSpec &g = segs[L.i];
+            // after p passes the data sits in fin iff (passes - p) is even
+            const bool in_fin = ((L.passes - p) & 1) == 0;
+            const void *src = in_fin ? g.fin : g.other;
+            void *dst = in_fin ? g.other : g.fin;
+            const uint64_t last = p == L.passes - 1 ? kSegLastBit : 0;
+            for (uint64_t o = 0; o < g.len; o += MT) desc.push_back({src, dst, g.len, o | last});
+        }
+        pass_cnt[p] = desc.size() - pass_start[p];
+    }
+    // one upload for every descriptor table
+    size_t bytes = 0;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t o_td[3];
+    for (int c = 0; c < 3; c++) {
+        o_td[c] = bytes;
+        bytes = align(bytes + td[c].size() * sizeof(TileDescH));
+    }
+    const size_t o_desc = bytes;
+    bytes = align(bytes + desc.size() * sizeof(MergeDescH));
+    if (bytes == 0) return OGS_OK;
+    Workspace *ws_ = workspace(s);
+    std::lock_guard<std::mutex> g(ws_->mu);
+    int rc = ws_->reserve(bytes);
+    if (rc) return rc;
+    // the previous upload from this staging buffer must have been consumed
+    e = hipEventSynchronize(ws_->staged);
+    if (e != hipSuccess) return hip_fail(e, "staging wait");
+    std::vector<uint8_t> &h = ws_->host;
+    h.resize(bytes);
+    for (int c = 0; c < 3; c++)
+        if (!td[c].empty()) memcpy(h.data() + o_td[c], td[c].data(), td[c].size() * sizeof(TileDescH));
+    if (!desc.empty()) memcpy(h.data() + o_desc, desc.data(), desc.size() * sizeof(MergeDescH));
+    e = hipMemcpyAsync(ws_->dev, h.data(), bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(ws_->staged, s);
+    if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+    uint8_t *D = (uint8_t *)ws_->dev;
+    const BlockShapeFns *cls[3] = {&tb, &sb, &lb};
+    for (int c = 0; c < 3; c++) {
+        e = cls[c]->run_seg(D + o_td[c], td[c].size(), ki.codec, ki.codec, s);
+        if (e != hipSuccess) return hip_fail(e, "batched tile-sort launch");
+    }
+    for (int p = 0; p < max_p; p++) {
+        e = mb.run_seg(D + o_desc + pass_start[p] * sizeof(MergeDescH), pass_cnt[p], LT << p, ki.codec, s);
+        if (e != hipSuccess) return hip_fail(e, "batched merge launch");
+    }
+    return OGS_OK;
+}
+
+// CSR form: segment i = [off[i], off[i+1]) of data; results land in scratch.
 int Ctx::segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg, void **out,
                    hipStream_t s) {
     KeyInfo ki;
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
     if (!out) return fail(OGS_ERR_ARG, "d_sorted is NULL");
     const uint64_t total = nseg ? h_off[nseg] : 0;
-    *out = scratch;
     if (total == 0) {
         *out = data;
         return OGS_OK;
     }
     if (!data || !scratch || data == scratch) return fail(OGS_ERR_ARG, "bad data/scratch buffers");
-    hipError_t e = hipSetDevice(device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    const BlockShapeFns &sb = block_shape(ki.kc, TP_SEG_SMALL_BLOCK);
-    const BlockShapeFns &lb = block_shape(ki.kc, TP_SEG_LARGE_BLOCK);
-    const MergeShapeFns &mb = merge_shape(ki.kc, TP_SEG_MERGE, lb.tile());
-    uint64_t small_max = (uint64_t)tune[ki.kc].v[TP_SEG_SMALL_MAX];
-    if (small_max == 0 || small_max > sb.tile()) small_max = sb.tile();
-    const uint64_t LT = lb.tile(), MT = mb.tile();
-
-    std::vector<uint64_t> s_off, l_off;
-    std::vector<uint32_t> s_cnt, l_cnt;
-    struct Large { uint64_t seg, len; int passes; };
-    std::vector<Large> large;
+    std::vector<SegSpec> segs(nseg);
+    const uint64_t w = (uint64_t)ki.width;
     for (uint64_t i = 0; i < nseg; i++) {
         const uint64_t a = h_off[i], b = h_off[i + 1];
-        if (b < a || b > total) return fail(OGS_ERR_ARG, "offsets not non-decreasing at segment %llu",
-                                            (unsigned long long)i);
-        const uint64_t len = b - a;
-        if (len == 0) continue;
-        if (len <= small_max) {
-            s_off.push_back(a);
-            s_cnt.push_back((uint32_t)len | kTileFinalBit);
-        } else {
-            int p = 0;
-            for (uint64_t R = LT; R < len; R *= 2) p++;
-            large.push_back({a, len, p});
-            const uint32_t flags = ((p & 1) ? kTileOddBit : 0u) | (p == 0 ? kTileFinalBit : 0u);
-            for (uint64_t o = 0; o < len; o += LT) {
-                l_off.push_back(a + o);
-                l_cnt.push_back((uint32_t)std::min<uint64_t>(LT, len - o) | flags);
-            }
-        }
+        if (b < a || b > total)
+            return fail(OGS_ERR_ARG, "offsets not non-decreasing at segment %llu", (unsigned long long)i);
+        segs[i] = {(char *)data + a * w, (char *)scratch + a * w, (char *)data + a * w, b - a};
     }
-    // merge-pass descriptors, pass by pass; segments with more passes left
-    // read from the buffer that their parity dictates
-    int max_p = 0;
-    for (auto &L : large) max_p = std::max(max_p, L.passes);
-    std::vector<SegTileDesc> desc;
-    std::vector<uint64_t> pass_start(max_p + 1, 0), pass_cnt(max_p + 1, 0);
-    for (int p = 0; p < max_p; p++) {
-        pass_start[p] = desc.size();
-        const uint64_t R = LT << p;
-        for (auto &L : large) {
-            if (L.passes <= p) continue;
-            // after p passes the data sits in scratch iff (passes - p) is even
-            const bool in_scratch = ((L.passes - p) & 1) == 0;
-            const uint64_t flags = (in_scratch ? kSegDirBit : 0) | (p == L.passes - 1 ? kSegLastBit : 0);
-            (void)R;
-            for (uint64_t o = 0; o < L.len; o += MT) desc.push_back({L.seg, L.len, o | flags});
-        }
-        pass_cnt[p] = desc.size() - pass_start[p];
-    }
-    // one device upload for every descriptor table
-    const size_t n_s = s_off.size(), n_l = l_off.size(), n_d = desc.size();
-    size_t bytes = 0;
-    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t o_soff = bytes; bytes = align(bytes + n_s * 8);
-    const size_t o_scnt = bytes; bytes = align(bytes + n_s * 4);
-    const size_t o_loff = bytes; bytes = align(bytes + n_l * 8);
-    const size_t o_lcnt = bytes; bytes = align(bytes + n_l * 4);
-    const size_t o_desc = bytes; bytes = align(bytes + n_d * sizeof(SegTileDesc));
-    const size_t o_end = bytes;
-    Workspace *w = workspace(s);
-    int rc = w->reserve(bytes);
-    if (rc) return rc;
-    std::vector<uint8_t> &h = w->host;
-    h.resize(o_end);
-    if (n_s) {
-        memcpy(h.data() + o_soff, s_off.data(), n_s * 8);
-        memcpy(h.data() + o_scnt, s_cnt.data(), n_s * 4);
-    }
-    if (n_l) {
-        memcpy(h.data() + o_loff, l_off.data(), n_l * 8);
-        memcpy(h.data() + o_lcnt, l_cnt.data(), n_l * 4);
-    }
-    if (n_d) memcpy(h.data() + o_desc, desc.data(), n_d * sizeof(SegTileDesc));
-    // pageable-source copy: returns once the source has been consumed
-    e = hipMemcpyAsync(w->dev, h.data(), o_end, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) return hip_fail(e, "descriptor upload");
-    uint8_t *D = (uint8_t *)w->dev;
-    e = sb.run_seg(data, scratch, data, (const uint64_t *)(D + o_soff), (const uint32_t *)(D + o_scnt), n_s,
-                   ki.codec, ki.codec, s);
-    if (e != hipSuccess) return hip_fail(e, "small-segment launch");
-    e = lb.run_seg(data, scratch, data, (const uint64_t *)(D + o_loff), (const uint32_t *)(D + o_lcnt), n_l,
-                   ki.codec, ki.codec, s);
-    if (e != hipSuccess) return hip_fail(e, "large-segment tile launch");
-    for (int p = 0; p < max_p; p++) {
-        e = mb.run_seg(data, scratch, D + o_desc + pass_start[p] * sizeof(SegTileDesc), pass_cnt[p], LT << p,
-                       ki.codec, s);
-        if (e != hipSuccess) return hip_fail(e, "segmented merge launch");
-    }
-    return OGS_OK;
+    *out = scratch;
+    return batch(kt, segs.data(), segs.size(), s);
 }
 
 }  // namespace ogs
@@ -351,6 +376,7 @@ int ogs_ctx_create(ogs_ctx **ctx, int device) {
     c->device = device;
     for (int k = 0; k < 3; k++) c->tune[k] = default_tuning((KeyClass)k);
     *ctx = reinterpret_cast<ogs_ctx *>(c);
+    load_default_tiles(*ctx);
     return OGS_OK;
 }
 
@@ -440,6 +466,22 @@ int ogs_segmented_sort(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch
         return fail(OGS_ERR_ARG, "offsets must start at 0 and end at n=%llu (got %llu..%llu)", (unsigned long long)n,
                     (unsigned long long)off[0], (unsigned long long)off[nseg]);
     return c->segmented(kt, d_data, d_scratch, off.data(), nseg, d_sorted, (hipStream_t)stream);
+}
+
+int ogs_sort_batch(ogs_ctx *ctx, ogs_key_t kt, const ogs_batch_item *items, uint64_t count, void *stream) {
+    if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
+    if (count && !items) return fail(OGS_ERR_ARG, "items is NULL");
+    std::vector<SegSpec> segs(count);
+    for (uint64_t i = 0; i < count; i++) {
+        const ogs_batch_item &it = items[i];
+        // a segment whose scratch is NULL may use its input as the second buffer
+        void *other = it.scratch ? it.scratch : (void *)it.in;
+        if (it.n && other == it.out)
+            return fail(OGS_ERR_ARG, "item %llu: scratch (or, without scratch, in) must differ from out",
+                        (unsigned long long)i);
+        segs[i] = {it.in, it.out, other, it.n};
+    }
+    return reinterpret_cast<Ctx *>(ctx)->batch(kt, segs.data(), segs.size(), (hipStream_t)stream);
 }
 
 int ogs_sort_host(ogs_ctx *ctx, ogs_key_t kt, void *h_data, uint64_t n, uint64_t chunk) {

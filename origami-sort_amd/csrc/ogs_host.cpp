@@ -1,9 +1,14 @@
 // ogs_host.cpp -- host-side parts of the C-ABI that launch nothing: the seeded
 // input generator (the datagen::Writer analogue, Origami/writer.h:47-242) and
 // the tile-table accessors used by the autotuner.
+#include <dlfcn.h>
+
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "ogs_internal.hpp"
@@ -19,8 +24,8 @@ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
     return z ^ (z >> 31);
 }
 
-const char *kParamNames[TP_COUNT] = {"plain_block", "plain_merge", "seg_small_block",
-                                     "seg_small_max", "seg_large_block", "seg_merge"};
+const char *kParamNames[TP_COUNT] = {"plain_block", "plain_merge", "seg_small_block", "seg_small_max",
+                                     "seg_large_block", "seg_merge", "seg_tiny_block", "seg_tiny_max"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -110,9 +115,10 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
     const int p = param_index(param);
     if (p < 0) return fail(OGS_ERR_ARG, "unknown tuning parameter '%s'", param ? param : "(null)");
     int cnt = 0;
+    const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX;
     if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
-    else if (p != TP_SEG_SMALL_MAX) block_shapes(ki.kc, &cnt);
-    if (p == TP_SEG_SMALL_MAX ? value < 0 : (value < 0 || value >= cnt))
+    else if (!is_count) block_shapes(ki.kc, &cnt);
+    if (is_count ? value < 0 : (value < 0 || value >= cnt))
         return fail(OGS_ERR_ARG, "value %d out of range for '%s'", value, param);
     reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
     return OGS_OK;
@@ -126,6 +132,36 @@ int ogs_tune_get(ogs_ctx *ctx, ogs_key_t kt, const char *param, int *value) {
     if (p < 0) return fail(OGS_ERR_ARG, "unknown tuning parameter '%s'", param ? param : "(null)");
     *value = reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p];
     return OGS_OK;
+}
+
+// Tile-table files: one "kind param value" triple per line, '#' comments.
+// kind is u32/i32/f32/u64/i64/kv_u64u64/kv_i64i64 (the classes share tables:
+// u32/i32/f32, u64/i64, kv_*).
+int ogs_tune_load(ogs_ctx *ctx, const char *path) {
+    if (!ctx || !path) return fail(OGS_ERR_ARG, "NULL argument");
+    FILE *f = fopen(path, "r");
+    if (!f) return fail(OGS_ERR_ARG, "cannot open tile table '%s'", path);
+    static const char *kinds[] = {"u32", "i32", "u64", "i64", "f32", "kv_u64u64", "kv_i64i64"};
+    char line[256];
+    int lineno = 0, rc = OGS_OK;
+    while (fgets(line, sizeof line, f)) {
+        lineno++;
+        char *h = strchr(line, '#');
+        if (h) *h = 0;
+        char k[64], p[64];
+        int v;
+        const int got = sscanf(line, "%63s %63s %d", k, p, &v);
+        if (got <= 0) continue;
+        if (got != 3) { rc = fail(OGS_ERR_ARG, "%s:%d: expected 'kind param value'", path, lineno); break; }
+        int kt = -1;
+        for (int i = 0; i < 7; i++)
+            if (!strcmp(k, kinds[i])) kt = i;
+        if (kt < 0) { rc = fail(OGS_ERR_ARG, "%s:%d: unknown kind '%s'", path, lineno, k); break; }
+        rc = ogs_tune_set(ctx, (ogs_key_t)kt, p, v);
+        if (rc) break;
+    }
+    fclose(f);
+    return rc;
 }
 
 // Compiled shapes: merge = 0 lists block-sort shapes (block, ipt, wave levels),
@@ -153,3 +189,26 @@ int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *
 }
 
 }  // extern "C"
+
+namespace ogs {
+// Applies $OGS_TILES, else ogs_tiles.txt beside libogs.so, when present.
+void load_default_tiles(ogs_ctx *ctx) {
+    const char *env = getenv("OGS_TILES");
+    std::string path;
+    if (env && *env) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (dladdr((void *)&ogs_tune_load, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/ogs_tiles.txt";
+        }
+    }
+    if (path.empty()) return;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return;
+    fclose(f);
+    (void)ogs_tune_load(ctx, path.c_str());
+}
+}  // namespace ogs

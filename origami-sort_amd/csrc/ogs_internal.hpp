@@ -25,6 +25,8 @@ bool key_info(ogs_key_t kt, KeyInfo *ki);
 
 // Tile table per key class: indices into the compiled shape lists (the
 // analogue of the tuned constants in Origami/config.h:45-264).
+// Segment size classes: tiny (one wavefront-sized workgroup per segment),
+// small (one workgroup per segment), large (TILE-sized pieces + merge passes).
 enum TuneParam {
     TP_PLAIN_BLOCK = 0,     // block-sort shape of plain sorts
     TP_PLAIN_MERGE = 1,     // merge shape of plain sorts
@@ -32,7 +34,9 @@ enum TuneParam {
     TP_SEG_SMALL_MAX = 3,   // longest segment in the small class (0 = its tile)
     TP_SEG_LARGE_BLOCK = 4, // block-sort shape of segments above the small class
     TP_SEG_MERGE = 5,       // merge shape of segmented passes
-    TP_COUNT = 6
+    TP_SEG_TINY_BLOCK = 6,  // block-sort shape of the tiny segment class
+    TP_SEG_TINY_MAX = 7,    // longest segment in the tiny class (0 = no tiny class)
+    TP_COUNT = 8
 };
 struct Tuning {
     int v[TP_COUNT];
@@ -42,11 +46,23 @@ struct Tuning {
 // on one stream is ordered, so reusing the block across calls on the same
 // stream is safe; growing it frees the old block (hipFree synchronises).
 struct Workspace {
+    std::mutex mu;  // one batched call at a time per stream
     void *dev = nullptr;
     size_t cap = 0;
-    std::vector<uint8_t> host;  // staging for descriptor uploads
+    std::vector<uint8_t> host;       // staging for descriptor uploads
+    hipEvent_t staged = nullptr;     // recorded after the last upload from `host`
     int reserve(size_t bytes);
+    Workspace();
     ~Workspace();
+};
+
+// One segment of a batched sort: sorted from `in` into `fin`, with `other`
+// as the second ping-pong buffer (`in` may alias either).
+struct SegSpec {
+    const void *in;
+    void *fin;
+    void *other;
+    uint64_t len;
 };
 
 // Per-kernel timing with HIP events recorded on the launching stream
@@ -88,7 +104,10 @@ struct Ctx {
     int sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s);
     int segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg, void **out,
                   hipStream_t s);
+    int batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s);
     int pass_count(ogs_key_t kt, uint64_t n);
 };
+
+void load_default_tiles(ogs_ctx *ctx);
 
 }  // namespace ogs

@@ -97,25 +97,27 @@ __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *__restrict__ i
     tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc);
 }
 
-// Segmented tile sort: descriptor i = {item offset, count} of one tile
-// (a whole small segment, or one TILE-sized piece of a large segment).
-// Bit 31 of tile_cnt[i] picks out_odd instead of out_even as the destination
-// of tile i, so a large segment starts on the buffer parity its pass count
-// needs (out_odd may equal `in`: a tile is read whole before it is written);
-// bit 30 marks a tile that is its segment's final output (undo the codec).
+// Batched tile sort: descriptor i sorts one tile -- a whole small segment, or
+// one TILE-sized piece of a large segment -- from `in` to `out` (they may be
+// the same buffer: a tile is read whole before it is written).  Bit 30 of
+// cnt marks a tile that is its segment's final output (undo the codec).
+struct TileDesc {
+    const void *in;
+    void *out;
+    uint32_t cnt;
+    uint32_t pad;
+};
+static_assert(sizeof(TileDesc) == 24, "descriptor layout is shared with the host");
+
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-__global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const void *in, void *out_even, void *out_odd,
-                                                         const uint64_t *__restrict__ tile_off,
-                                                         const uint32_t *__restrict__ tile_cnt, int cin, int cout) {
+__global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const TileDesc *__restrict__ tiles, int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K sk[TILE];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ int smisc[2];
-    const uint64_t base = tile_off[blockIdx.x];
-    const uint32_t c = tile_cnt[blockIdx.x];
-    const int cnt = (int)(c & 0x3FFFFFFFu);
-    void *out = (c >> 31) ? out_odd : out_even;
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, ((c >> 30) & 1u) ? cout : 0, sk, sv, smisc);
+    const TileDesc t = tiles[blockIdx.x];
+    const int cnt = (int)(t.cnt & 0x3FFFFFFFu);
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, ((t.cnt >> 30) & 1u) ? cout : 0, sk, sv, smisc);
 }
 
 // ----------------------------------------------------------------------------
@@ -197,29 +199,30 @@ __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ i
     merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco);
 }
 
-// Segmented form: descriptor t = {segment base, segment length, tile start
-// within the segment}; one entry per output tile of every segment that still
-// has runs of length R to merge.  Bit 63 of o0 is the direction: 0 reads buf0
-// and writes buf1, 1 the reverse (segments with different pass counts sit on
-// different buffers in the same pass); bit 62 marks the segment's last pass.
-struct SegTile {
-    uint64_t seg, len, o0;
+// Batched form: descriptor t = one output tile of one segment that still has
+// runs of length R to merge: the segment's source and destination buffers,
+// its length, and the tile start (bit 62 marks the segment's last pass, when
+// the output codec is undone).  Segments with different pass counts simply
+// carry different buffers.
+struct MergeDesc {
+    const void *src;
+    void *dst;
+    uint64_t len;
+    uint64_t o0;
 };
-static constexpr uint64_t kDirBit = 1ull << 63;
+static_assert(sizeof(MergeDesc) == 32, "descriptor layout is shared with the host");
 static constexpr uint64_t kLastBit = 1ull << 62;
 
 template <typename K, bool HV, int BLOCK, int IPT>
-__global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(void *buf0, void *buf1, const SegTile *__restrict__ tiles,
-                                                          uint64_t R, int cout) {
+__global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__restrict__ tiles, uint64_t R, int cout) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K sk[TILE];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ uint64_t sco[2];
-    const SegTile d = tiles[blockIdx.x];
-    const bool dir = (d.o0 & kDirBit) != 0;
+    const MergeDesc d = tiles[blockIdx.x];
     const int co = (d.o0 & kLastBit) ? cout : 0;
-    const uint64_t o0 = d.o0 & ~(kDirBit | kLastBit);
-    merge_tile<K, HV, BLOCK, IPT>(dir ? buf1 : buf0, dir ? buf0 : buf1, d.seg, d.len, R, o0, co, sk, sv, sco);
+    const uint64_t o0 = d.o0 & ~kLastBit;
+    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco);
 }
 
 // ----------------------------------------------------------------------------
@@ -237,11 +240,10 @@ static hipError_t launch_block(const void *in, void *out, uint64_t n, int cin, i
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-static hipError_t launch_seg_block(const void *in, void *out_even, void *out_odd, const uint64_t *toff,
-                                   const uint32_t *tcnt, uint64_t ntiles, int cin, int cout, hipStream_t s) {
+static hipError_t launch_seg_block(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_seg_tile_sort<K, HV, BLOCK, IPT, WLEV>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in,
-                       out_even, out_odd, toff, tcnt, cin, cout);
+    hipLaunchKernelGGL((k_seg_tile_sort<K, HV, BLOCK, IPT, WLEV>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s,
+                       (const TileDesc *)tiles, cin, cout);
     return hipGetLastError();
 }
 
@@ -256,11 +258,10 @@ static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R
 }
 
 template <typename K, bool HV, int BLOCK, int IPT>
-static hipError_t launch_seg_merge(void *buf0, void *buf1, const void *tiles, uint64_t ntiles, uint64_t R, int cout,
-                                   hipStream_t s) {
+static hipError_t launch_seg_merge(const void *tiles, uint64_t ntiles, uint64_t R, int cout, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_seg_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, buf0, buf1,
-                       (const SegTile *)tiles, R, cout);
+    hipLaunchKernelGGL((k_seg_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s,
+                       (const MergeDesc *)tiles, R, cout);
     return hipGetLastError();
 }
 

@@ -9,25 +9,30 @@ namespace ogs {
 // Key classes: the kernels only see ordered unsigned keys.
 enum KeyClass { KC_32 = 0, KC_64 = 1, KC_KV = 2 };
 
-// Per-tile descriptor of the segmented merge pass (must match the device
-// struct): segment base, segment length, tile start; bit 63 of o0 = direction
-// (1: scratch -> data), bit 62 = the segment's last pass.
-struct SegTileDesc {
-    uint64_t seg, len, o0;
+// Batched-sort descriptors (layouts shared with the device structs).
+// Tile sort: sort `cnt & 0x3FFFFFFF` items from `in` to `out`; bit 30 of cnt
+// = final output of its segment (undo the key codec).
+struct TileDescH {
+    const void *in;
+    void *out;
+    uint32_t cnt;
+    uint32_t pad;
 };
-static constexpr uint64_t kSegDirBit = 1ull << 63;
+// Merge pass: one output tile of a segment: src/dst buffers of the segment,
+// segment length, tile start (bit 62 = the segment's last pass).
+struct MergeDescH {
+    const void *src;
+    void *dst;
+    uint64_t len;
+    uint64_t o0;
+};
 static constexpr uint64_t kSegLastBit = 1ull << 62;
-// Tile-sort descriptor count word: bit 31 = write to the odd buffer, bit 30 =
-// final output of its segment, bits 0..29 = item count.
-static constexpr uint32_t kTileOddBit = 1u << 31;
 static constexpr uint32_t kTileFinalBit = 1u << 30;
 
 typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s);
-typedef hipError_t (*SegBlockLaunch)(const void *in, void *out_even, void *out_odd, const uint64_t *tile_off,
-                                     const uint32_t *tile_cnt, uint64_t ntiles, int cin, int cout, hipStream_t s);
+typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s);
 typedef hipError_t (*MergeLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, hipStream_t s);
-typedef hipError_t (*SegMergeLaunch)(void *buf0, void *buf1, const void *tiles, uint64_t ntiles, uint64_t R, int cout,
-                                     hipStream_t s);
+typedef hipError_t (*SegMergeLaunch)(const void *tiles, uint64_t ntiles, uint64_t R, int cout, hipStream_t s);
 
 struct BlockShapeFns {
     int block, ipt, wlev;

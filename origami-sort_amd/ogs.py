@@ -52,10 +52,12 @@ def lib():
         L.ogs_profile_reset.argtypes = [vp]
         L.ogs_segmented_sort.argtypes = [vp, i, vp, vp, vp, u64, u64, ctypes.POINTER(vp), vp]
         L.ogs_sort_host.argtypes = [vp, i, vp, u64, u64]
+        L.ogs_sort_batch.argtypes = [vp, i, vp, u64, vp]
         L.ogs_pass_count.argtypes = [vp, i, u64]
         L.ogs_generate_host.argtypes = [i, i, u64, u64, vp]
         L.ogs_tune_set.argtypes = [vp, i, cp, i]
         L.ogs_tune_get.argtypes = [vp, i, cp, ctypes.POINTER(i)]
+        L.ogs_tune_load.argtypes = [vp, cp]
         L.ogs_shape_info.argtypes = [i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
         L.ogs_last_error.restype = cp
         _lib = L
@@ -84,6 +86,11 @@ def shapes(kind, merge):
         L.ogs_shape_info(kind, merge, idx, ctypes.byref(b), ctypes.byref(p), ctypes.byref(w))
         res.append((b.value, p.value, w.value))
     return res
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("in_", ctypes.c_void_p), ("out", ctypes.c_void_p), ("scratch", ctypes.c_void_p),
+                ("n", ctypes.c_uint64)]
 
 
 class Context:
@@ -141,16 +148,28 @@ class Context:
                                        ctypes.c_void_p(offsets), nseg, n, ctypes.byref(out), ctypes.c_void_p(stream)))
         return out.value
 
+    def sort_batch_ptrs(self, kind, items, stream=0):
+        """items: list of (in_ptr, out_ptr, scratch_ptr_or_0, n)."""
+        arr = (BatchItem * len(items))(*[BatchItem(a, b, c or None, n) for a, b, c, n in items])
+        check(lib().ogs_sort_batch(self.h, kind, ctypes.cast(arr, ctypes.c_void_p), len(items),
+                                   ctypes.c_void_p(stream)))
+
     def sort_host(self, kind, arr, chunk=0):
         a = np.ascontiguousarray(arr)
         check(lib().ogs_sort_host(self.h, kind, ctypes.c_void_p(a.ctypes.data), a.size, chunk))
         return a
+
+    TUNE_PARAMS = ("plain_block", "plain_merge", "seg_tiny_block", "seg_tiny_max", "seg_small_block",
+                   "seg_small_max", "seg_large_block", "seg_merge")
 
     def pass_count(self, kind, n):
         return check(lib().ogs_pass_count(self.h, kind, n))
 
     def tune_set(self, kind, param, value):
         check(lib().ogs_tune_set(self.h, kind, param.encode(), value))
+
+    def tune_load(self, path):
+        check(lib().ogs_tune_load(self.h, path.encode()))
 
     def tune_get(self, kind, param):
         v = ctypes.c_int()

@@ -197,3 +197,62 @@ def test_every_compiled_shape(ctx, kind):
     finally:
         ctx.tune_set(kind, "plain_block", 0)
         ctx.tune_set(kind, "plain_merge", 0)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.KV_I64I64, ogs.F32])
+def test_pointer_batch(ctx, kind):
+    # ogs_sort_batch: independent arrays, out-of-place, in-place (out == in) and
+    # without scratch (in clobbered), all in one launch sequence
+    import torch
+    say(f"case pointer batch kind={ogs.KIND_NAMES[kind]}")
+    rng = np.random.default_rng(2)
+    lens = [0, 1, 17, 600, 4096, 4097, 70000, 300001] + [int(x) for x in rng.integers(1, 40000, 20)]
+    arrays, items, keep = [], [], []
+    for j, n in enumerate(lens):
+        a = ogs.generate(kind, ogs.NORMAL if kind == ogs.F32 else ogs.UNIFORM, n, seed=100 + j)
+        arrays.append(a)
+        din = ogs.to_device(a) if n else torch.zeros(16, dtype=torch.uint8, device="cuda")
+        mode = j % 3
+        if mode == 0:
+            out, scr = torch.empty_like(din), torch.empty_like(din)
+            items.append((din.data_ptr(), out.data_ptr(), scr.data_ptr(), n))
+        elif mode == 1:
+            out, scr = din, torch.empty_like(din)
+            items.append((din.data_ptr(), din.data_ptr(), scr.data_ptr(), n))
+        else:
+            out, scr = torch.empty_like(din), None
+            items.append((din.data_ptr(), out.data_ptr(), 0, n))
+        keep.append((din, out, scr))
+    ctx.sort_batch_ptrs(kind, items)
+    torch.cuda.synchronize()
+    for a, (din, out, scr), n in zip(arrays, keep, lens):
+        if n == 0:
+            continue
+        assert_same(kind, ogs.from_device(out, kind, n), a)
+
+
+def test_segmented_tiny_class(ctx):
+    # the wavefront-sized class (one 64-lane workgroup per segment) with a cut-off
+    import torch
+    kind = ogs.U32
+    bshapes = ogs.shapes(kind, 0)
+    tiny = [i for i, s in enumerate(bshapes) if s[0] == 64][0]
+    ctx.tune_set(kind, "seg_tiny_block", tiny)
+    ctx.tune_set(kind, "seg_tiny_max", 300)
+    try:
+        say("case segmented tiny class")
+        rng = np.random.default_rng(4)
+        lens = rng.integers(0, 2000, 3000)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+        n = int(off[-1])
+        a = ogs.generate(kind, ogs.UNIFORM, n, seed=8)
+        d = ogs.to_device(a)
+        s = torch.empty_like(d)
+        doff = torch.from_numpy(off.view(np.int64).copy()).cuda()
+        p = ctx.segmented_ptr(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+        torch.cuda.synchronize()
+        got = ogs.from_device(d if p == d.data_ptr() else s, kind, n)
+        ref = oracle.segmented_sort(kind, a, off, threads=4)
+        assert np.array_equal(got, ref)
+    finally:
+        ctx.tune_set(kind, "seg_tiny_max", 0)
