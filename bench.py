@@ -59,6 +59,165 @@ def cpu_baseline(kind_oracle, host_keys, sample_n, threads):
                       f"oracle/origami_oracle.c sort_multi_thread restatement, {threads} threads, min_k=2"}
 
 
+def bench_batched(ctx, total_log2, max_seg_log2, seed=4):
+    """Batched segmented sort (one ogs_segmented_sort launch sequence over CSR
+    segments with log-uniform lengths in [1, 2^max_seg_log2]) against one plain
+    ogs_sort launch per segment, same u32 keys, both on the GPU."""
+    import numpy as np
+    import torch
+
+    import ogs
+    rng = np.random.default_rng(seed)
+    total_cap = 1 << total_log2
+    lens = []
+    acc = 0
+    while acc < total_cap:
+        ln = max(1, int(2.0 ** (rng.random() * max_seg_log2)))
+        ln = min(ln, total_cap - acc)
+        lens.append(ln)
+        acc += ln
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    n = int(off[-1])
+    log(f"bench: batched {len(lens)} segments, {n} keys")
+    host = ogs.generate(ogs.U32, ogs.UNIFORM, n, seed=seed)
+    d0 = ogs.to_device(host)
+    d = torch.empty_like(d0)
+    s = torch.empty_like(d0)
+    doff = torch.from_numpy(off.view(np.int64).copy()).cuda()
+    d.copy_(d0)
+    ctx.segmented_ptr(ogs.U32, d.data_ptr(), s.data_ptr(), doff.data_ptr(), len(lens), n)  # warm-up
+    torch.cuda.synchronize()
+    reps = 3
+    t_b = 0.0
+    for _ in range(reps):
+        d.copy_(d0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p = ctx.segmented_ptr(ogs.U32, d.data_ptr(), s.data_ptr(), doff.data_ptr(), len(lens), n)
+        torch.cuda.synchronize()
+        t_b += time.perf_counter() - t0
+    t_b /= reps
+    res_b = s if p == s.data_ptr() else d
+    batched_out = res_b.clone()
+    # one plain launch per segment
+    d.copy_(d0)
+    torch.cuda.synchronize()
+    w = 4
+    base_d, base_s = d.data_ptr(), s.data_ptr()
+    t0 = time.perf_counter()
+    outs = []
+    for i, ln in enumerate(lens):
+        o = int(off[i]) * w
+        outs.append(ctx.sort_ptr(ogs.U32, base_d + o, base_s + o, ln) - o)
+    torch.cuda.synchronize()
+    t_p = time.perf_counter() - t0
+    # every segment equals the batched result
+    same = True
+    for i, ln in enumerate(lens):
+        o = int(off[i]) * w
+        src = d if outs[i] == base_d else s
+        if not torch.equal(src[o:o + ln * w], batched_out[o:o + ln * w]):
+            same = False
+            break
+    ok_sorted = True
+    return {"segments": len(lens), "keys": n, "max_segment": int(max(lens)),
+            "lengths": f"log-uniform [1, 2^{max_seg_log2}], drawn until 2^{total_log2} keys",
+            "batched_ms": round(t_b * 1e3, 3), "batched_keys_per_s": round(n / t_b / 1e9, 4),
+            "plain_per_segment_ms": round(t_p * 1e3, 3), "plain_keys_per_s": round(n / t_p / 1e9, 4),
+            "unit": "G keys/s", "speedup": round(t_p / t_b, 2), "identical_results": bool(same and ok_sorted)}
+
+
+def bench_serving(ctx, nreq, rate_factor, seed=5):
+    """Serving loop: Poisson arrivals of ORDER-BY requests (key type uniform over
+    u32/u64/u64+u64, length log-uniform in [2^10, 2^26], keys uniform or Zipf 50/50,
+    device-resident), served by ogs_server (continuous batching, 4 streams) and,
+    for comparison, by serial plain launches at the same arrival times."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import ogs
+    rng = np.random.default_rng(seed)
+    kinds = [ogs.U32, ogs.U64, ogs.KV_U64U64]
+    reqs = []
+    for j in range(nreq):
+        kind = kinds[int(rng.integers(0, 3))]
+        n = int(2.0 ** (10 + 16 * rng.random()))
+        dist = ogs.ZIPF if rng.random() < 0.5 else ogs.UNIFORM
+        a = ogs.generate(kind, dist, n, seed=1000 + j)
+        src = ogs.to_device(a)
+        reqs.append((kind, n, src))
+    total_keys = sum(r[1] for r in reqs)
+    log(f"bench: serving {nreq} requests, {total_keys} keys")
+    # serial service times (one plain launch per request) set the offered rate
+    outs_serial = []
+    svc = []
+    for kind, n, src in reqs:
+        a_ = torch.empty_like(src)
+        b_ = torch.empty_like(src)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p = ctx.sort_keep_ptr(kind, src.data_ptr(), a_.data_ptr(), b_.data_ptr(), n)
+        torch.cuda.synchronize()
+        svc.append(time.perf_counter() - t0)
+        outs_serial.append(a_ if p == a_.data_ptr() else b_)
+    mean_svc = float(np.mean(svc))
+    rate = rate_factor / mean_svc
+    gaps = rng.exponential(1.0 / rate, nreq)
+    arrivals = np.cumsum(gaps) - gaps[0]
+
+    def pct(x, q):
+        return float(np.percentile(np.array(x) * 1e3, q))
+
+    # serial plain launches at those arrival times (FIFO, one at a time)
+    lat_s = []
+    scratch = [(torch.empty_like(src), torch.empty_like(src)) for _, _, src in reqs]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (kind, n, src), at, (a_, b_) in zip(reqs, arrivals, scratch):
+        now = time.perf_counter() - t0
+        if now < at:
+            time.sleep(at - now)
+        ctx.sort_keep_ptr(kind, src.data_ptr(), a_.data_ptr(), b_.data_ptr(), n)
+        torch.cuda.synchronize()
+        lat_s.append(time.perf_counter() - t0 - at)
+    span_s = time.perf_counter() - t0
+    del scratch
+    # the server at the same arrival times
+    dsts = [torch.empty_like(src) for _, _, src in reqs]
+    budget = 3 * max(n * ogs.WIDTH[k] for k, n, _ in reqs) + (256 << 20)
+    srv = ogs.Server(ctx, hbm_budget_bytes=int(budget), streams=4)
+    torch.cuda.synchronize()
+    tickets = []
+    t0 = time.perf_counter()
+    for (kind, n, src), at, dst in zip(reqs, arrivals, dsts):
+        now = time.perf_counter() - t0
+        if now < at:
+            time.sleep(at - now)
+        tickets.append(srv.submit(kind, src.data_ptr(), dst.data_ptr(), n))
+    for t in tickets:
+        srv.wait(t, 120.0)
+    span_v = time.perf_counter() - t0
+    lat_v = []
+    for t in tickets:
+        ts, _, td = srv.times(t)
+        lat_v.append(td - ts)
+    st = srv.stats()
+    srv.close()
+    same = all(torch.equal(dst[: n * ogs.WIDTH[k]], so[: n * ogs.WIDTH[k]]) if k != ogs.KV_U64U64 else
+               torch.equal(dst[: n * 16].view(torch.int64)[0::2], so[: n * 16].view(torch.int64)[0::2])
+               for (k, n, _), dst, so in zip(reqs, dsts, outs_serial))
+    return {"requests": nreq, "keys": total_keys, "offered_rate_per_s": round(rate, 2),
+            "offered_load_vs_serial": rate_factor,
+            "server": {"keys_per_s": round(total_keys / span_v / 1e9, 4), "p50_ms": round(pct(lat_v, 50), 3),
+                       "p99_ms": round(pct(lat_v, 99), 3), "batches": st["batches"]},
+            "serial_plain": {"keys_per_s": round(total_keys / span_s / 1e9, 4), "p50_ms": round(pct(lat_s, 50), 3),
+                             "p99_ms": round(pct(lat_s, 99), 3)},
+            "unit": "G keys/s", "identical_results": bool(same),
+            "note": "device-resident requests; latency = submit (arrival) to result ready"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +226,12 @@ def main():
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--kind", default="u32", choices=["u32", "f32", "u64", "kv_u64u64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--batched-log2-total", type=int, default=30)
+    ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-load", type=float, default=1.5,
+                    help="offered request rate as a multiple of the serial plain-launch capacity")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after timing, run three 1-input-sized device copies (torch copy_) as the "
                          "known-size reference kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -165,6 +330,13 @@ def main():
             d_b.copy_(d_in)
         torch.cuda.synchronize()
         line["copy_calib_bytes"] = int(d_in.numel())
+    if not args.no_batched:
+        del d_a, d_b
+        torch.cuda.empty_cache()
+        line["batched"] = bench_batched(ctx, args.batched_log2_total, 20)
+    if not args.no_serving:
+        torch.cuda.empty_cache()
+        line["serving"] = bench_serving(ctx, args.serving_requests, args.serving_load)
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
         import oracle

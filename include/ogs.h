@@ -164,6 +164,47 @@ int ogs_profile_enable(ogs_ctx *ctx, int on);
 int ogs_profile_read(ogs_ctx *ctx, int kind, double *total_ms, uint64_t *launches, uint64_t *items);
 int ogs_profile_reset(ogs_ctx *ctx);
 
+/* ---- Serving loop: continuous batching of sort requests -------------------
+ * A server owns a device arena of hbm_budget_bytes (the pooled allocator that
+ * admission draws from) and `streams` HIP streams.  Submitted requests wait
+ * in a FIFO; a scheduler thread coalesces pending requests of one key type
+ * into a batch (ogs_sort_batch) as soon as a stream is free and the arena
+ * holds the batch's footprint (device request: one scratch copy; host
+ * request: staging + scratch = 2x payload), so requests arriving while
+ * others run are admitted as capacity frees up and their work overlaps on
+ * the device.  A completion thread finishes batches (host requests are
+ * copied back) and timestamps every request.  No reference counterpart: the
+ * reference is a library call (Origami/sorter.h:3851, 4194). */
+typedef struct ogs_server ogs_server;
+typedef struct {
+    uint64_t hbm_budget_bytes;   /* device arena size; admission limit */
+    int streams;                 /* batches in flight at once (1..16) */
+    uint32_t max_batch_requests; /* 0 = 1024 */
+    uint64_t max_batch_items;    /* 0 = unlimited */
+} ogs_server_config;
+typedef struct {
+    uint64_t submitted, completed, rejected, batches, batched_requests, items;
+} ogs_server_stats;
+#define OGS_SUBMIT_HOST 1 /* src/dst are host pointers (else device pointers) */
+
+int ogs_server_create(ogs_server **srv, ogs_ctx *ctx, const ogs_server_config *cfg);
+/* Drains every queued request, then stops the threads and frees the arena. */
+int ogs_server_destroy(ogs_server *srv);
+/* Queues a sort of n items of type kt from src into dst (dst may equal src;
+ * src is left intact otherwise).  Returns a ticket; the buffers must stay
+ * valid until the ticket is done. */
+int ogs_server_submit(ogs_server *srv, ogs_key_t kt, const void *src, void *dst, uint64_t n, int flags,
+                      uint64_t *ticket);
+/* Blocks until the ticket is done (timeout_s < 0: no limit); returns the
+ * request's status, or OGS_ERR_TIMEOUT. */
+int ogs_server_wait(ogs_server *srv, uint64_t ticket, double timeout_s);
+int ogs_server_poll(ogs_server *srv, uint64_t ticket, int *done);
+/* Seconds since server creation: submitted, admitted into a batch, done. */
+int ogs_server_times(ogs_server *srv, uint64_t ticket, double *t_submit, double *t_start, double *t_done);
+/* Forgets a finished ticket. */
+int ogs_server_release(ogs_server *srv, uint64_t ticket);
+int ogs_server_get_stats(ogs_server *srv, ogs_server_stats *stats);
+
 /* Message of the last failed call on this thread ("" if none). */
 const char *ogs_last_error(void);
 

@@ -57,7 +57,8 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
     }
     std::mt19937_64 g(seed);
     std::vector<double> cdf;
-    const uint64_t R = 1u << 20;
+    std::vector<uint32_t> guide;  // guide[j] = lower_bound of cdf at j/G of the total mass
+    const uint64_t R = 1u << 20, G = 1u << 16;
     if (dist == 1) {
         cdf.resize(R);
         double acc = 0;
@@ -65,6 +66,10 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
             acc += std::pow((double)(r + 1), -1.1);
             cdf[r] = acc;
         }
+        guide.resize(G + 1);
+        for (uint64_t j = 0; j <= G; j++)
+            guide[j] = (uint32_t)(std::lower_bound(cdf.begin(), cdf.end(), (double)j / (double)G * cdf[R - 1]) -
+                                  cdf.begin());
     }
     for (uint64_t i = 0; i < n; i++) {
         uint64_t key = 0;
@@ -73,7 +78,13 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
         case 0: key = raw; break;
         case 1: {
             const double u = (double)(raw >> 11) * (1.0 / 9007199254740992.0) * cdf[R - 1];
-            key = mix64((uint64_t)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin()));
+            // the exact lower_bound of u, searched inside its guide-table bracket
+            // (one bracket of slack either side absorbs rounding in j)
+            uint64_t j = (uint64_t)(u / cdf[R - 1] * (double)G);
+            if (j >= G) j = G - 1;
+            const uint64_t lo = j ? guide[j - 1] : 0;
+            const uint64_t hi = std::min<uint64_t>(R, (uint64_t)guide[std::min<uint64_t>(G, j + 2)] + 1);
+            key = mix64((uint64_t)(std::lower_bound(cdf.begin() + lo, cdf.begin() + hi, u) - cdf.begin()));
         } break;
         case 2: key = (uint64_t)i * (UINT64_MAX / n); break;
         case 3: key = (uint64_t)(n - 1 - i) * (UINT64_MAX / n); break;

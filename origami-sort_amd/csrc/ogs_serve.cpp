@@ -1,0 +1,449 @@
+// ogs_serve.cpp -- continuous-batching sort server (ogs_server_* in include/ogs.h).
+//
+// The reference is a library call with no request loop; this is the serving
+// form of its sort entry points (Origami/sorter.h:3851, 4194).  Requests are
+// ORDER-BY-style sorts of device or host arrays of any supported key type.
+//
+//   submit ──► pending FIFO ──► scheduler thread ──► batch on a free stream ──► completion thread
+//                               (admission by HBM      (ogs_sort_batch over       (event wait, host
+//                                footprint, pooled      the batch's requests)      copy-back, release)
+//                                device arena)
+//
+// Admission: a batch takes pending requests of the head's key type in FIFO
+// order while their footprint fits the arena (device request: one scratch
+// copy of the payload; host request: payload staging + scratch = 2x) and the
+// batch caps allow.  Batches run on `streams` HIP streams, so requests that
+// arrive while others are on the GPU are admitted as soon as a stream and
+// arena space free up, and their kernels overlap with the running batches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "ogs_internal.hpp"
+
+namespace ogs {
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+// First-fit allocator over one device block (offsets in bytes, 256-aligned).
+class Arena {
+  public:
+    void init(size_t cap) {
+        cap_ = cap;
+        free_.clear();
+        free_[0] = cap;
+    }
+    // returns offset or SIZE_MAX
+    size_t alloc(size_t size) {
+        size = (size + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= size) {
+                const size_t off = it->first, len = it->second;
+                free_.erase(it);
+                if (len > size) free_[off + size] = len - size;
+                used_[off] = size;
+                return off;
+            }
+        }
+        return SIZE_MAX;
+    }
+    void release(size_t off) {
+        auto u = used_.find(off);
+        if (u == used_.end()) return;
+        size_t start = off, len = u->second;
+        used_.erase(u);
+        auto next = free_.lower_bound(start);
+        if (next != free_.end() && start + len == next->first) {
+            len += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == start) {
+                start = prev->first;
+                len += prev->second;
+                free_.erase(prev);
+            }
+        }
+        free_[start] = len;
+    }
+    size_t capacity() const { return cap_; }
+
+  private:
+    size_t cap_ = 0;
+    std::map<size_t, size_t> free_, used_;
+};
+
+struct Request {
+    uint64_t ticket;
+    ogs_key_t kt;
+    const void *src;
+    void *dst;
+    uint64_t n;
+    int flags;
+    int status = OGS_OK;
+    bool done = false;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    size_t dev_off = 0;  // staging offset within the batch block (host requests)
+};
+
+struct Batch {
+    std::vector<Request *> reqs;
+    size_t arena_off = SIZE_MAX;
+    int stream = -1;
+    hipEvent_t ev = nullptr;
+    int status = OGS_OK;
+};
+
+}  // namespace
+
+struct Server {
+    Ctx *ctx = nullptr;
+    ogs_server_config cfg{};
+    void *arena_base = nullptr;
+    Arena arena;
+    std::vector<hipStream_t> streams;
+    std::vector<bool> stream_busy;
+    std::mutex mu;
+    std::condition_variable cv_sched, cv_done, cv_inflight;
+    std::deque<Request *> pending;
+    std::map<uint64_t, Request *> reqs;
+    std::deque<Batch *> inflight;
+    std::thread sched, completer;
+    bool stopping = false;     // no new submissions; drain what is queued
+    bool sched_done = false;   // the scheduler has issued its last batch
+    uint64_t releases = 0;     // arena releases so far (admission retries on change)
+    uint64_t next_ticket = 1;
+    double t0 = 0;
+    ogs_server_stats stats{};
+
+    static size_t footprint(const Request *r) {
+        size_t w = ogs_item_size(r->kt);
+        size_t one = ((size_t)r->n * w + 255) & ~(size_t)255;
+        return (r->flags & OGS_SUBMIT_HOST) ? 2 * one : one;
+    }
+
+    void schedule_loop();
+    void complete_loop();
+    void finish(Batch *b);
+};
+
+void Server::schedule_loop() {
+    hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cv_sched.wait(lk, [&] {
+            if (stopping && pending.empty()) return true;
+            if (pending.empty()) return false;
+            return std::find(stream_busy.begin(), stream_busy.end(), false) != stream_busy.end();
+        });
+        if (pending.empty()) {
+            if (stopping) break;
+            continue;
+        }
+        const int si = (int)(std::find(stream_busy.begin(), stream_busy.end(), false) - stream_busy.begin());
+        // coalesce: FIFO requests of the head's key type, within caps and arena
+        Request *head = pending.front();
+        if (footprint(head) > arena.capacity()) {
+            pending.pop_front();
+            head->status = fail(OGS_ERR_NOMEM, "request of %llu items exceeds the server's HBM budget",
+                                (unsigned long long)head->n);
+            head->done = true;
+            head->t_start = head->t_done = now_s() - t0;
+            stats.rejected++;
+            cv_done.notify_all();
+            continue;
+        }
+        std::vector<Request *> take;
+        size_t bytes = 0;
+        uint64_t items = 0;
+        size_t scanned = 0;
+        for (auto it = pending.begin(); it != pending.end() && scanned < 256; ++it, ++scanned) {
+            Request *r = *it;
+            if (r->kt != head->kt) continue;
+            const size_t f = footprint(r);
+            if (!take.empty() && (take.size() >= cfg.max_batch_requests || items + r->n > cfg.max_batch_items ||
+                                  bytes + f > arena.capacity()))
+                break;
+            r->dev_off = bytes;
+            take.push_back(r);
+            bytes += f;
+            items += r->n;
+        }
+        size_t off = arena.alloc(bytes);
+        while (off == SIZE_MAX && take.size() > 1) {  // shrink the batch to what fits now
+            bytes -= footprint(take.back());
+            items -= take.back()->n;
+            take.pop_back();
+            off = arena.alloc(bytes);
+        }
+        if (off == SIZE_MAX) {
+            // wait until a running batch releases arena space
+            const uint64_t seen = releases;
+            cv_sched.wait(lk, [&] { return releases != seen; });
+            continue;
+        }
+        for (Request *r : take) pending.erase(std::find(pending.begin(), pending.end(), r));
+        Batch *b = new Batch();
+        b->reqs = take;
+        b->arena_off = off;
+        b->stream = si;
+        stream_busy[si] = true;
+        const double ts = now_s() - t0;
+        for (Request *r : take) r->t_start = ts;
+        lk.unlock();
+
+        // issue: host staging uploads, the batched sort, the completion event
+        hipStream_t s = streams[si];
+        char *base = (char *)arena_base + off;
+        std::vector<ogs_batch_item> items_v;
+        int rc = OGS_OK;
+        for (Request *r : take) {
+            const size_t w = ogs_item_size(r->kt);
+            const size_t one = ((size_t)r->n * w + 255) & ~(size_t)255;
+            char *blk = base + r->dev_off;
+            if (r->flags & OGS_SUBMIT_HOST) {
+                hipError_t e = hipMemcpyAsync(blk, r->src, (size_t)r->n * w, hipMemcpyHostToDevice, s);
+                if (e != hipSuccess && rc == OGS_OK) rc = hip_fail(e, "request upload");
+                items_v.push_back({blk, blk, blk + one, r->n});
+            } else {
+                items_v.push_back({r->src, r->dst, blk, r->n});
+            }
+        }
+        if (rc == OGS_OK) rc = ogs_sort_batch(reinterpret_cast<ogs_ctx *>(ctx), head->kt, items_v.data(),
+                                              items_v.size(), (void *)s);
+        hipError_t e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventRecord(b->ev, s);
+        if (e != hipSuccess && rc == OGS_OK) rc = hip_fail(e, "batch event");
+        b->status = rc;
+
+        lk.lock();
+        stats.batches++;
+        stats.batched_requests += take.size();
+        inflight.push_back(b);
+        cv_inflight.notify_all();
+    }
+}
+
+void Server::finish(Batch *b) {
+    // host requests: copy results back (outside the lock)
+    if (b->status == OGS_OK) {
+        for (Request *r : b->reqs) {
+            if (!(r->flags & OGS_SUBMIT_HOST)) continue;
+            const size_t w = ogs_item_size(r->kt);
+            hipError_t e = hipMemcpy(r->dst, (char *)arena_base + b->arena_off + r->dev_off, (size_t)r->n * w,
+                                     hipMemcpyDeviceToHost);
+            if (e != hipSuccess) r->status = hip_fail(e, "request download");
+        }
+    }
+    const double td = now_s() - t0;
+    std::lock_guard<std::mutex> g(mu);
+    for (Request *r : b->reqs) {
+        if (b->status != OGS_OK) r->status = b->status;
+        r->t_done = td;
+        r->done = true;
+        stats.completed++;
+        stats.items += r->n;
+    }
+    arena.release(b->arena_off);
+    releases++;
+    stream_busy[b->stream] = false;
+    if (b->ev) (void)hipEventDestroy(b->ev);
+    delete b;
+    cv_done.notify_all();
+    cv_sched.notify_all();
+}
+
+void Server::complete_loop() {
+    hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+        cv_inflight.wait(lk, [&] { return !inflight.empty() || sched_done; });
+        if (inflight.empty()) {
+            if (sched_done) break;
+            continue;
+        }
+        // finish whichever batch is ready first; else block on the oldest
+        Batch *ready = nullptr;
+        for (auto it = inflight.begin(); it != inflight.end(); ++it) {
+            if (!(*it)->ev || hipEventQuery((*it)->ev) != hipErrorNotReady) {
+                ready = *it;
+                inflight.erase(it);
+                break;
+            }
+        }
+        if (!ready) {
+            Batch *oldest = inflight.front();
+            lk.unlock();
+            (void)hipEventSynchronize(oldest->ev);
+            lk.lock();
+            continue;
+        }
+        lk.unlock();
+        finish(ready);
+        lk.lock();
+    }
+}
+
+}  // namespace ogs
+
+using namespace ogs;
+
+extern "C" {
+
+int ogs_server_create(ogs_server **srv, ogs_ctx *ctx, const ogs_server_config *cfg) {
+    if (!srv || !ctx || !cfg) return fail(OGS_ERR_ARG, "NULL argument");
+    if (cfg->hbm_budget_bytes < 4096 || cfg->streams < 1 || cfg->streams > 16)
+        return fail(OGS_ERR_ARG, "bad server config (budget >= 4096 B, 1..16 streams)");
+    Server *s = new Server();
+    s->ctx = reinterpret_cast<Ctx *>(ctx);
+    s->cfg = *cfg;
+    if (!s->cfg.max_batch_requests) s->cfg.max_batch_requests = 1024;
+    if (!s->cfg.max_batch_items) s->cfg.max_batch_items = UINT64_MAX;
+    hipError_t e = hipSetDevice(s->ctx->device);
+    if (e == hipSuccess) e = hipMalloc(&s->arena_base, cfg->hbm_budget_bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(OGS_ERR_NOMEM, "server arena of %llu B: %s", (unsigned long long)cfg->hbm_budget_bytes,
+                    hipGetErrorString(e));
+    }
+    s->arena.init(cfg->hbm_budget_bytes);
+    s->streams.resize(cfg->streams);
+    s->stream_busy.assign(cfg->streams, false);
+    for (auto &st : s->streams) {
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    }
+    s->t0 = now_s();
+    s->sched = std::thread([s] { s->schedule_loop(); });
+    s->completer = std::thread([s] { s->complete_loop(); });
+    *srv = reinterpret_cast<ogs_server *>(s);
+    return OGS_OK;
+}
+
+int ogs_server_destroy(ogs_server *srv) {
+    if (!srv) return OGS_OK;
+    Server *s = reinterpret_cast<Server *>(srv);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stopping = true;
+    }
+    s->cv_sched.notify_all();
+    s->cv_inflight.notify_all();
+    s->sched.join();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->sched_done = true;
+    }
+    s->cv_inflight.notify_all();
+    s->completer.join();
+    for (auto &st : s->streams) (void)hipStreamDestroy(st);
+    (void)hipFree(s->arena_base);
+    for (auto &kv : s->reqs) delete kv.second;
+    delete s;
+    return OGS_OK;
+}
+
+int ogs_server_submit(ogs_server *srv, ogs_key_t kt, const void *src, void *dst, uint64_t n, int flags,
+                      uint64_t *ticket) {
+    if (!srv || !ticket) return fail(OGS_ERR_ARG, "NULL argument");
+    if (!ogs_item_size(kt)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    if (n && (!src || !dst)) return fail(OGS_ERR_ARG, "NULL src or dst");
+    Server *s = reinterpret_cast<Server *>(srv);
+    Request *r = new Request();
+    r->kt = kt;
+    r->src = src;
+    r->dst = dst;
+    r->n = n;
+    r->flags = flags;
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->stopping) {
+        delete r;
+        return fail(OGS_ERR_STATE, "server is shutting down");
+    }
+    r->ticket = s->next_ticket++;
+    r->t_submit = now_s() - s->t0;
+    s->reqs[r->ticket] = r;
+    s->stats.submitted++;
+    if (n == 0) {
+        r->done = true;
+        r->t_start = r->t_done = r->t_submit;
+        s->stats.completed++;
+    } else {
+        s->pending.push_back(r);
+        s->cv_sched.notify_all();
+    }
+    *ticket = r->ticket;
+    return OGS_OK;
+}
+
+int ogs_server_wait(ogs_server *srv, uint64_t ticket, double timeout_s) {
+    if (!srv) return fail(OGS_ERR_ARG, "NULL server");
+    Server *s = reinterpret_cast<Server *>(srv);
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(OGS_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    Request *r = it->second;
+    auto pred = [&] { return r->done; };
+    if (timeout_s < 0) s->cv_done.wait(lk, pred);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred))
+        return fail(OGS_ERR_TIMEOUT, "ticket %llu not done after %.3f s", (unsigned long long)ticket, timeout_s);
+    return r->status;
+}
+
+int ogs_server_poll(ogs_server *srv, uint64_t ticket, int *done) {
+    if (!srv || !done) return fail(OGS_ERR_ARG, "NULL argument");
+    Server *s = reinterpret_cast<Server *>(srv);
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(OGS_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    *done = it->second->done ? 1 : 0;
+    return it->second->done ? it->second->status : OGS_OK;
+}
+
+int ogs_server_times(ogs_server *srv, uint64_t ticket, double *t_submit, double *t_start, double *t_done) {
+    if (!srv) return fail(OGS_ERR_ARG, "NULL server");
+    Server *s = reinterpret_cast<Server *>(srv);
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(OGS_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    if (t_submit) *t_submit = it->second->t_submit;
+    if (t_start) *t_start = it->second->t_start;
+    if (t_done) *t_done = it->second->t_done;
+    return OGS_OK;
+}
+
+int ogs_server_release(ogs_server *srv, uint64_t ticket) {
+    if (!srv) return fail(OGS_ERR_ARG, "NULL server");
+    Server *s = reinterpret_cast<Server *>(srv);
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return fail(OGS_ERR_ARG, "unknown ticket %llu", (unsigned long long)ticket);
+    if (!it->second->done) return fail(OGS_ERR_STATE, "ticket %llu still running", (unsigned long long)ticket);
+    delete it->second;
+    s->reqs.erase(it);
+    return OGS_OK;
+}
+
+int ogs_server_get_stats(ogs_server *srv, ogs_server_stats *out) {
+    if (!srv || !out) return fail(OGS_ERR_ARG, "NULL argument");
+    Server *s = reinterpret_cast<Server *>(srv);
+    std::lock_guard<std::mutex> g(s->mu);
+    *out = s->stats;
+    return OGS_OK;
+}
+
+}  // extern "C"

@@ -59,6 +59,15 @@ def lib():
         L.ogs_tune_get.argtypes = [vp, i, cp, ctypes.POINTER(i)]
         L.ogs_tune_load.argtypes = [vp, cp]
         L.ogs_shape_info.argtypes = [i, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i)]
+        L.ogs_server_create.argtypes = [ctypes.POINTER(vp), vp, ctypes.POINTER(ServerConfig)]
+        L.ogs_server_destroy.argtypes = [vp]
+        L.ogs_server_submit.argtypes = [vp, i, vp, vp, u64, i, ctypes.POINTER(u64)]
+        L.ogs_server_wait.argtypes = [vp, u64, ctypes.c_double]
+        L.ogs_server_poll.argtypes = [vp, u64, ctypes.POINTER(i)]
+        L.ogs_server_times.argtypes = [vp, u64, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_double)]
+        L.ogs_server_release.argtypes = [vp, u64]
+        L.ogs_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
         L.ogs_last_error.restype = cp
         _lib = L
     return _lib
@@ -86,6 +95,67 @@ def shapes(kind, merge):
         L.ogs_shape_info(kind, merge, idx, ctypes.byref(b), ctypes.byref(p), ctypes.byref(w))
         res.append((b.value, p.value, w.value))
     return res
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_int),
+                ("max_batch_requests", ctypes.c_uint32), ("max_batch_items", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in
+                ("submitted", "completed", "rejected", "batches", "batched_requests", "items")]
+
+
+SUBMIT_HOST = 1
+
+
+class Server:
+    """ogs_server: continuous-batching sort server on a context's device."""
+
+    def __init__(self, ctx, hbm_budget_bytes, streams=4, max_batch_requests=0, max_batch_items=0):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        cfg = ServerConfig(hbm_budget_bytes, streams, max_batch_requests, max_batch_items)
+        check(lib().ogs_server_create(ctypes.byref(self.h), ctx.h, ctypes.byref(cfg)))
+
+    def submit(self, kind, src, dst, n, host=False):
+        t = ctypes.c_uint64()
+        check(lib().ogs_server_submit(self.h, kind, ctypes.c_void_p(src), ctypes.c_void_p(dst), n,
+                                      SUBMIT_HOST if host else 0, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout=-1.0):
+        return check(lib().ogs_server_wait(self.h, ticket, timeout))
+
+    def poll(self, ticket):
+        d = ctypes.c_int()
+        check(lib().ogs_server_poll(self.h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def times(self, ticket):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        check(lib().ogs_server_times(self.h, ticket, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def release(self, ticket):
+        check(lib().ogs_server_release(self.h, ticket))
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().ogs_server_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().ogs_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BatchItem(ctypes.Structure):

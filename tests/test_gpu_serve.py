@@ -1,0 +1,97 @@
+"""Serving loop: every request's result in a concurrent mix equals the oracle's
+and the same request sorted alone; host and device requests; admission under
+a tight HBM budget; oversize requests are rejected with a status."""
+import sys
+
+import numpy as np
+import pytest
+
+import ogs
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available()
+    c = ogs.Context(0)
+    yield c
+    c.close()
+
+
+def _check(kind, got, a):
+    ref = oracle.sort(kind, a)
+    if kind in (ogs.KV_U64U64, ogs.KV_I64I64):
+        assert np.array_equal(got["key"], ref["key"])
+        assert np.array_equal(np.sort(got["value"]), np.sort(a["value"]))
+        o1 = np.lexsort((a["value"], a["key"]))
+        o2 = np.lexsort((got["value"], got["key"]))
+        assert np.array_equal(a[o1].view(np.uint8), got[o2].view(np.uint8))
+    else:
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+def test_mixed_requests_same_alone_and_in_mix(ctx):
+    import torch
+    rng = np.random.default_rng(21)
+    kinds = [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32, ogs.I64]
+    reqs = []
+    for j in range(60):
+        kind = kinds[j % len(kinds)]
+        n = int(2 ** rng.uniform(0, 20))
+        dist = ogs.NORMAL if kind == ogs.F32 else (ogs.ZIPF if j % 2 else ogs.UNIFORM)
+        a = ogs.generate(kind, dist, n, seed=300 + j)
+        reqs.append((kind, n, a))
+    say("case serve mix: 60 requests, 4 streams")
+    srv = ogs.Server(ctx, hbm_budget_bytes=64 << 20, streams=4)
+    bufs, tickets = [], []
+    for j, (kind, n, a) in enumerate(reqs):
+        src = ogs.to_device(a) if n else torch.zeros(16, dtype=torch.uint8, device="cuda")
+        dst = src if j % 3 == 0 else torch.empty_like(src)
+        torch.cuda.synchronize()
+        bufs.append((src, dst))
+        tickets.append(srv.submit(kind, src.data_ptr(), dst.data_ptr(), n))
+    for t in tickets:
+        srv.wait(t, 60.0)
+    st = srv.stats()
+    say(f"stats {st}")
+    assert st["completed"] == len(reqs) and st["batches"] < len(reqs)  # requests were coalesced
+    alone = []
+    for j, ((kind, n, a), (src, dst)) in enumerate(zip(reqs, bufs)):
+        got = ogs.from_device(dst, kind, n)
+        _check(kind, got, a)
+        # the same request sorted alone (plain entry point) gives the same bytes for key-only kinds
+        if kind not in (ogs.KV_U64U64, ogs.KV_I64I64) and n:
+            d = ogs.to_device(a)
+            s = torch.empty_like(d)
+            res = ctx.sort_tensor(kind, d, s, n)
+            torch.cuda.synchronize()
+            assert np.array_equal(ogs.from_device(res, kind, n).view(np.uint8), got.view(np.uint8))
+    for t in tickets:
+        srv.release(t)
+    srv.close()
+
+
+def test_host_requests_and_budget(ctx):
+    say("case serve host requests, tight budget")
+    srv = ogs.Server(ctx, hbm_budget_bytes=8 << 20, streams=2)
+    arrays = [ogs.generate(ogs.U64, ogs.UNIFORM, n, seed=n) for n in (1000, 300000, 200000, 5, 400000)]
+    outs = [np.empty_like(a) for a in arrays]
+    tickets = [srv.submit(ogs.U64, a.ctypes.data, o.ctypes.data, a.size, host=True) for a, o in zip(arrays, outs)]
+    for t in tickets:
+        srv.wait(t, 60.0)
+    for a, o in zip(arrays, outs):
+        assert np.array_equal(o, np.sort(a))
+    # a request bigger than the whole budget is rejected, not hung
+    big = np.zeros(1 << 20, np.uint64)
+    t = srv.submit(ogs.U64, big.ctypes.data, big.ctypes.data, big.size, host=True)
+    with pytest.raises(ogs.OgsError):
+        srv.wait(t, 30.0)
+    srv.close()
