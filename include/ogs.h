@@ -136,7 +136,9 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
 /* Tile table (the run-time counterpart of the tuned constants in
  * Origami/config.h:45-264).  param: "plain_block", "plain_merge",
  * "seg_tiny_block", "seg_tiny_max", "seg_small_block", "seg_small_max",
- * "seg_large_block", "seg_merge".  Block and merge parameters take an index
+ * "seg_large_block", "seg_merge", "plain_corank" (0: a partition kernel
+ * computes every merge tile's co-rank first, 1: each merge workgroup searches
+ * its own).  Block and merge parameters take an index
  * into ogs_shape_info's lists; *_max take an item count.  Segments of up to
  * seg_tiny_max items (0 = no tiny class) get one workgroup of the tiny shape
  * each, segments up to seg_small_max (0 = the small shape's tile) one

@@ -75,17 +75,20 @@ def tune_class(ctx, name, kind, quick):
     best = None
     for bi, (bb, bp, bw) in enumerate(bs):
         for mi, (mb, mp, _) in enumerate(ms):
-            if mb * mp > 2 * bb * bp:
+            if mb * mp > 2 * bb * bp or bb * bp < 2048 or mb * mp < 2048:
                 continue
-            ctx.tune_set(kind, "plain_block", bi)
-            ctx.tune_set(kind, "plain_merge", mi)
-            t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
-            log(f"  {name} plain n={n} block={bs[bi]} merge={ms[mi]}: {t * 1e3:.3f} ms")
-            if best is None or t < best[0]:
-                best = (t, bi, mi)
+            for corank in (0, 1):
+                ctx.tune_set(kind, "plain_block", bi)
+                ctx.tune_set(kind, "plain_merge", mi)
+                ctx.tune_set(kind, "plain_corank", corank)
+                t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
+                log(f"  {name} plain n={n} block={bs[bi]} merge={ms[mi]} corank={corank}: {t * 1e3:.3f} ms")
+                if best is None or t < best[0]:
+                    best = (t, bi, mi, corank)
     ctx.tune_set(kind, "plain_block", best[1])
     ctx.tune_set(kind, "plain_merge", best[2])
-    result["plain_block"], result["plain_merge"] = best[1], best[2]
+    ctx.tune_set(kind, "plain_corank", best[3])
+    result["plain_block"], result["plain_merge"], result["plain_corank"] = best[1], best[2], best[3]
     result["_plain"] = {"n": n, "ms": best[0] * 1e3, "gkeys": n / best[0] / 1e9}
     del d_in, a, b
     # ---- segmented batch: log-uniform lengths in [1, 2^20]

@@ -64,6 +64,7 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_SEG_MERGE] = 0;
     t.v[TP_SEG_TINY_BLOCK] = 0;
     t.v[TP_SEG_TINY_MAX] = 0;
+    t.v[TP_PLAIN_CORANK] = 0;
     return t;
 }
 
@@ -112,10 +113,14 @@ const MergeShapeFns &Ctx::merge_shape(KeyClass kc, int param, uint64_t block_til
     const MergeShapeFns *t = merge_shapes(kc, &cnt);
     int i = tune[kc].v[param];
     if (i < 0 || i >= cnt) i = 0;
-    // a merge tile must divide every pair span 2R (R >= block tile)
+    // a merge tile must divide every pair span 2R (R >= block tile): otherwise
+    // take the largest registered merge tile that does (every class registers
+    // a 512-item shape, which fits the smallest block tile)
     if (t[i].tile() > 2 * block_tile) {
+        int best = -1;
         for (int j = 0; j < cnt; j++)
-            if (t[j].tile() <= 2 * block_tile) return t[j];
+            if (t[j].tile() <= 2 * block_tile && (best < 0 || t[j].tile() > t[best].tile())) best = j;
+        if (best >= 0) return t[best];
     }
     return t[i];
 }
@@ -198,6 +203,14 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
         (void)hipEventRecord(ev[1], s);
         prof.pending.push_back({PK_BLOCK, ev[0], ev[1], n, true});
     }
+    uint64_t *part = nullptr;
+    if (!one && tune[ki.kc].v[TP_PLAIN_CORANK] == 0) {
+        Workspace *w = workspace(s);
+        std::lock_guard<std::mutex> g(w->mu);
+        int rc = w->reserve(((n + ms.tile() - 1) / ms.tile()) * sizeof(uint64_t));
+        if (rc) return rc;
+        part = (uint64_t *)w->dev;
+    }
     if (!one) {
         for (uint64_t R = T; R < n; R *= 2) {
             const bool last = 2 * R >= n;
@@ -206,7 +219,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
                 ev[1] = prof.get();
                 (void)hipEventRecord(ev[0], s);
             }
-            e = ms.run(src, dst, n, R, last ? ki.codec : 0, s);
+            e = ms.run(src, dst, n, R, last ? ki.codec : 0, part, s);
             if (e != hipSuccess) return hip_fail(e, "merge pass launch");
             if (prof.on) {
                 (void)hipEventRecord(ev[1], s);

@@ -24,8 +24,9 @@ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
     return z ^ (z >> 31);
 }
 
-const char *kParamNames[TP_COUNT] = {"plain_block", "plain_merge", "seg_small_block", "seg_small_max",
-                                     "seg_large_block", "seg_merge", "seg_tiny_block", "seg_tiny_max"};
+const char *kParamNames[TP_COUNT] = {"plain_block",     "plain_merge", "seg_small_block",
+                                     "seg_small_max",   "seg_large_block", "seg_merge",
+                                     "seg_tiny_block",  "seg_tiny_max", "plain_corank"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -128,6 +129,7 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
     int cnt = 0;
     const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX;
     if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
+    else if (p == TP_PLAIN_CORANK) cnt = 2;
     else if (!is_count) block_shapes(ki.kc, &cnt);
     if (is_count ? value < 0 : (value < 0 || value >= cnt))
         return fail(OGS_ERR_ARG, "value %d out of range for '%s'", value, param);

@@ -36,7 +36,8 @@ enum TuneParam {
     TP_SEG_MERGE = 5,       // merge shape of segmented passes
     TP_SEG_TINY_BLOCK = 6,  // block-sort shape of the tiny segment class
     TP_SEG_TINY_MAX = 7,    // longest segment in the tiny class (0 = no tiny class)
-    TP_COUNT = 8
+    TP_PLAIN_CORANK = 8,    // plain merges: 0 partition kernel + table, 1 co-ranks inside the merge kernel
+    TP_COUNT = 9
 };
 struct Tuning {
     int v[TP_COUNT];

@@ -137,14 +137,61 @@ __device__ __forceinline__ PairGeom pair_geom(uint64_t o, uint64_t n, uint64_t R
     return g;
 }
 
-// Merges one output tile: waves 0 and 1 find the co-ranks of the tile's start
-// and end, the tile's A and B slices are loaded into LDS as one coalesced
-// stream, each thread merges IPT outputs along its merge path, and the tile is
-// written back through LDS with lane-consecutive stores.
+// Co-rank of every merge tile's start, LANES lanes per tile: each step tests
+// LANES evenly spaced candidates of the merge-path predicate at once and keeps
+// the bracket the group's ballot bits point to (a 2^27-item run resolves in
+// seven dependent steps with 16 lanes).  part[j] = A items among the first
+// j*tile outputs of tile j's pair.
+template <typename K, bool HV, int LANES>
+__global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint64_t R, uint32_t tile,
+                                  uint64_t ntiles, uint64_t *__restrict__ part) {
+    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)(gt % LANES);
+    const int gshift = (int)(threadIdx.x & 63) & ~(LANES - 1);
+    uint64_t j = gt / LANES;
+    const bool live = j < ntiles;
+    if (!live) j = ntiles - 1;
+    const uint64_t o = j * tile;
+    const PairGeom g = pair_geom(o, n, R);
+    const uint64_t diag = o - g.pb;
+    const uint64_t abase = g.pb, bbase = g.pb + g.la;
+    uint64_t lo = diag > g.lb ? diag - g.lb : 0;
+    uint64_t hi = diag < g.la ? diag : g.la;
+    while (hi - lo > (uint64_t)LANES) {
+        const uint64_t span = hi - lo;
+        const uint64_t m = lo + (span * (uint64_t)(l + 1)) / (LANES + 1);
+        const K a = GMem<K, HV>::load_key(in, abase + m);
+        const K b = GMem<K, HV>::load_key(in, bbase + diag - 1 - m);
+        const uint64_t bits = (__ballot(b < a) >> gshift) & ((1ull << LANES) - 1);
+        if (!bits) {
+            lo = lo + (span * LANES) / (LANES + 1) + 1;
+        } else {
+            const int f = __ffsll((unsigned long long)bits) - 1;
+            const uint64_t mf = lo + (span * (uint64_t)(f + 1)) / (LANES + 1);
+            lo = f ? lo + (span * (uint64_t)f) / (LANES + 1) + 1 : lo;
+            hi = mf;
+        }
+    }
+    bool p = false;
+    if ((uint64_t)l < hi - lo) {
+        const uint64_t m = lo + l;
+        p = GMem<K, HV>::load_key(in, bbase + diag - 1 - m) < GMem<K, HV>::load_key(in, abase + m);
+    }
+    const uint64_t bits = (__ballot(p) >> gshift) & ((1ull << LANES) - 1);
+    const uint64_t a = bits ? lo + (uint64_t)(__ffsll((unsigned long long)bits) - 1) : hi;
+    if (live && l == 0) part[j] = a;
+}
+
+// Merges one output tile: the co-ranks of the tile's start and end come from
+// the partition table (part != nullptr: part[0] start, part[1] next tile's
+// start) or, without a table, from waves 0 and 1 searching in place.  The
+// tile's A and B slices are loaded into LDS as one coalesced stream, each
+// thread merges IPT outputs along its merge path, and the tile is written back
+// through LDS with lane-consecutive stores.
 template <typename K, bool HV, int BLOCK, int IPT>
 __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__restrict__ out, uint64_t seg,
                                            uint64_t n, uint64_t R, uint64_t o0, int cout, K *sk, uint64_t *sv,
-                                           uint64_t *sco) {
+                                           uint64_t *sco, const uint64_t *part) {
     static_assert(BLOCK >= 128, "the co-rank searches need two waves");
     constexpr int TILE = BLOCK * IPT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -152,15 +199,23 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
     const PairGeom g = pair_geom(o0, n, R);
     const uint64_t d0 = o0 - g.pb, d1 = d0 + cnt;
     const uint64_t abase = seg + g.pb, bbase = seg + g.pb + g.la;
-    if (wave == 0) {
-        const uint64_t a = wave_corank<K, HV>(in, abase, g.la, bbase, g.lb, d0, lane);
-        if (lane == 0) sco[0] = a;
-    } else if (wave == 1) {
-        const uint64_t a = (d1 == g.la + g.lb) ? g.la : wave_corank<K, HV>(in, abase, g.la, bbase, g.lb, d1, lane);
-        if (lane == 0) sco[1] = a;
+    uint64_t a0, a1;
+    if (part) {
+        a0 = part[0];
+        a1 = (d1 == g.la + g.lb) ? g.la : part[1];
+    } else {
+        if (wave == 0) {
+            const uint64_t a = wave_corank<K, HV>(in, abase, g.la, bbase, g.lb, d0, lane);
+            if (lane == 0) sco[0] = a;
+        } else if (wave == 1) {
+            const uint64_t a =
+                (d1 == g.la + g.lb) ? g.la : wave_corank<K, HV>(in, abase, g.la, bbase, g.lb, d1, lane);
+            if (lane == 0) sco[1] = a;
+        }
+        __syncthreads();
+        a0 = sco[0];
+        a1 = sco[1];
     }
-    __syncthreads();
-    const uint64_t a0 = sco[0], a1 = sco[1];
     const int la = (int)(a1 - a0);
     const int lb = cnt - la;
     const uint64_t A = abase + a0;
@@ -191,12 +246,13 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
 
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
-                                                      uint64_t R, int cout) {
+                                                      uint64_t R, int cout, const uint64_t *__restrict__ part) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K sk[TILE];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ uint64_t sco[2];
-    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco);
+    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco,
+                                  part ? part + blockIdx.x : nullptr);
 }
 
 // Batched form: descriptor t = one output tile of one segment that still has
@@ -222,7 +278,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__res
     const MergeDesc d = tiles[blockIdx.x];
     const int co = (d.o0 & kLastBit) ? cout : 0;
     const uint64_t o0 = d.o0 & ~kLastBit;
-    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco);
+    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco, nullptr);
 }
 
 // ----------------------------------------------------------------------------
@@ -247,13 +303,22 @@ static hipError_t launch_seg_block(const void *tiles, uint64_t ntiles, int cin, 
     return hipGetLastError();
 }
 
+// part == nullptr: co-ranks found inside the merge kernel; else `part` (ntiles
+// entries) is filled by the partition kernel first.
 template <typename K, bool HV, int BLOCK, int IPT>
-static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, int cout, hipStream_t s) {
+static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part,
+                               hipStream_t s) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
+    constexpr int LANES = 16;
     const uint64_t ntiles = (n + TILE - 1) / TILE;
     if (ntiles == 0) return hipSuccess;
+    if (part) {
+        const uint64_t threads = ntiles * LANES;
+        hipLaunchKernelGGL((k_merge_partition<K, HV, LANES>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                           s, in, n, R, (uint32_t)TILE, ntiles, part);
+    }
     hipLaunchKernelGGL((k_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n, R,
-                       cout);
+                       cout, (const uint64_t *)part);
     return hipGetLastError();
 }
 
@@ -270,13 +335,14 @@ static hipError_t launch_seg_merge(const void *tiles, uint64_t ntiles, uint64_t 
 // is the default; the tuning table may pick any other.
 #define OGS_BLOCK_SHAPES_32(X) X(256, 16, 1) X(256, 16, 0) X(256, 16, 2) X(256, 16, 3) X(256, 32, 1) \
     X(512, 16, 1) X(512, 32, 1) X(1024, 16, 1) X(1024, 32, 1) X(128, 16, 1) X(256, 8, 1) X(64, 16, 1) X(64, 8, 1)
-#define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8) X(1024, 16)
+#define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8) X(1024, 16) \
+    X(512, 32) X(128, 4)
 #define OGS_BLOCK_SHAPES_64(X) X(256, 16, 1) X(256, 8, 1) X(128, 16, 1) X(512, 8, 1) X(512, 16, 1) \
     X(1024, 16, 1) X(64, 16, 1) X(64, 8, 1)
-#define OGS_MERGE_SHAPES_64(X) X(256, 16) X(256, 8) X(512, 8) X(128, 16) X(512, 16)
+#define OGS_MERGE_SHAPES_64(X) X(256, 16) X(256, 8) X(512, 8) X(128, 16) X(512, 16) X(1024, 8) X(128, 4)
 #define OGS_BLOCK_SHAPES_KV(X) X(256, 8, 1) X(128, 8, 1) X(256, 4, 1) X(512, 4, 1) X(512, 8, 1) X(64, 8, 1) \
     X(64, 4, 1)
-#define OGS_MERGE_SHAPES_KV(X) X(256, 8) X(128, 8) X(256, 4) X(512, 4) X(512, 8)
+#define OGS_MERGE_SHAPES_KV(X) X(256, 8) X(128, 8) X(256, 4) X(512, 4) X(512, 8) X(1024, 4) X(128, 4)
 
 #define OGS_BX(K, HV, B, I, W) {B, I, W, &launch_block<K, HV, B, I, W>, &launch_seg_block<K, HV, B, I, W>},
 #define OGS_MX(K, HV, B, I) {B, I, &launch_merge<K, HV, B, I>, &launch_seg_merge<K, HV, B, I>},

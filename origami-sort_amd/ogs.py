@@ -229,8 +229,8 @@ class Context:
         check(lib().ogs_sort_host(self.h, kind, ctypes.c_void_p(a.ctypes.data), a.size, chunk))
         return a
 
-    TUNE_PARAMS = ("plain_block", "plain_merge", "seg_tiny_block", "seg_tiny_max", "seg_small_block",
-                   "seg_small_max", "seg_large_block", "seg_merge")
+    TUNE_PARAMS = ("plain_block", "plain_merge", "plain_corank", "seg_tiny_block", "seg_tiny_max",
+                   "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge")
 
     def pass_count(self, kind, n):
         return check(lib().ogs_pass_count(self.h, kind, n))

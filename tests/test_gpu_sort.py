@@ -188,15 +188,18 @@ def test_every_compiled_shape(ctx, kind):
                 a = ogs.generate(kind, ogs.UNIFORM if n % 2 else ogs.DISTINCT16, n, seed=n)
                 assert_same(kind, gpu_sort(ctx, kind, a), a)
         for mi, (b, ipt, w) in enumerate(mshapes):
-            ctx.tune_set(kind, "plain_block", 0)
-            ctx.tune_set(kind, "plain_merge", mi)
-            n = 200003
-            say(f"case merge shape kind={ogs.KIND_NAMES[kind]} merge={mshapes[mi]} n={n}")
-            a = ogs.generate(kind, ogs.ZIPF, n, seed=mi)
-            assert_same(kind, gpu_sort(ctx, kind, a), a)
+            for corank in (0, 1):
+                ctx.tune_set(kind, "plain_block", 0)
+                ctx.tune_set(kind, "plain_merge", mi)
+                ctx.tune_set(kind, "plain_corank", corank)
+                n = 200003
+                say(f"case merge shape kind={ogs.KIND_NAMES[kind]} merge={mshapes[mi]} corank={corank} n={n}")
+                a = ogs.generate(kind, ogs.ZIPF, n, seed=mi)
+                assert_same(kind, gpu_sort(ctx, kind, a), a)
     finally:
         ctx.tune_set(kind, "plain_block", 0)
         ctx.tune_set(kind, "plain_merge", 0)
+        ctx.tune_set(kind, "plain_corank", 0)
 
 
 @pytest.mark.parametrize("kind", [ogs.U32, ogs.KV_I64I64, ogs.F32])
