@@ -37,6 +37,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise OgsError(f"{LIB_PATH} is not built; run origami-sort_amd/build.sh")
+        # One HIP runtime per process: torch ships its own libamdhip64 (soname
+        # libamdhip64.so.7, file libamdhip64.so).  Importing torch first makes
+        # libogs.so bind to that copy; loading libogs.so first would pull in
+        # /opt/rocm's copy as well, and torch's device init then fails.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         vp, u64, i, cp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p
         L.ogs_ctx_create.argtypes = [ctypes.POINTER(vp), i]
