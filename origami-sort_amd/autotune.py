@@ -137,6 +137,7 @@ def tune_class(ctx, name, kind, quick):
         t_best = descend("seg_tiny_max", sorted({0, tt // 8, tt // 4, tt // 2, tt}))
     else:
         t_best = measure("seg_tiny_max", 0)
+    t_best = descend("seg_plain_min", [0, 18, 20, 22])
     result["_segmented"] = {"segments": nseg, "items": N, "ms": t_best * 1e3, "gkeys": N / t_best / 1e9}
     return result
 

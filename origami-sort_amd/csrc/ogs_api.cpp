@@ -65,6 +65,7 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_SEG_TINY_BLOCK] = 0;
     t.v[TP_SEG_TINY_MAX] = 0;
     t.v[TP_PLAIN_CORANK] = 0;
+    t.v[TP_SEG_PLAIN_MIN] = 22;
     return t;
 }
 
@@ -93,10 +94,27 @@ int Workspace::reserve(size_t bytes) {
     return OGS_OK;
 }
 
+int Workspace::reserve_part(size_t bytes) {
+    if (bytes <= part_cap) return OGS_OK;
+    size_t want = std::max(bytes, part_cap * 2);
+    want = std::max<size_t>(want, 1 << 20);
+    if (part) {
+        hipError_t e = hipFree(part);  // implicit device sync: earlier users of the block are done
+        part = nullptr;
+        part_cap = 0;
+        if (e != hipSuccess) return hip_fail(e, "hipFree(co-rank table)");
+    }
+    hipError_t e = hipMalloc(&part, want);
+    if (e != hipSuccess) return fail(OGS_ERR_NOMEM, "hipMalloc(co-rank table %zu B): %s", want, hipGetErrorString(e));
+    part_cap = want;
+    return OGS_OK;
+}
+
 Workspace::Workspace() { (void)hipEventCreateWithFlags(&staged, hipEventDisableTiming); }
 
 Workspace::~Workspace() {
     if (dev) (void)hipFree(dev);
+    if (part) (void)hipFree(part);
     if (staged) (void)hipEventDestroy(staged);
 }
 
@@ -173,7 +191,8 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     }
     if (!data || !scratch) return fail(OGS_ERR_ARG, "NULL data or scratch pointer");
     if (data == scratch) return fail(OGS_ERR_ARG, "data and scratch must be distinct buffers");
-    if (in && (in == data || in == scratch)) return fail(OGS_ERR_ARG, "input must differ from both work buffers");
+    // `in` may alias data or scratch: the block sort reads each tile whole
+    // before writing it back
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     const BlockShapeFns &bs = block_shape(ki.kc, TP_PLAIN_BLOCK);
@@ -207,23 +226,25 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     if (!one && tune[ki.kc].v[TP_PLAIN_CORANK] == 0) {
         Workspace *w = workspace(s);
         std::lock_guard<std::mutex> g(w->mu);
-        int rc = w->reserve(((n + ms.tile() - 1) / ms.tile()) * sizeof(uint64_t));
+        int rc = w->reserve_part(((n + ms.tile() - 1) / ms.tile()) * sizeof(uint64_t));
         if (rc) return rc;
-        part = (uint64_t *)w->dev;
+        part = (uint64_t *)w->part;
     }
     if (!one) {
         for (uint64_t R = T; R < n; R *= 2) {
             const bool last = 2 * R >= n;
             if (prof.on) {
-                ev[0] = prof.get();
-                ev[1] = prof.get();
+                for (int k = 0; k < 3; k++) ev[k] = prof.get();
                 (void)hipEventRecord(ev[0], s);
             }
-            e = ms.run(src, dst, n, R, last ? ki.codec : 0, part, s);
+            e = ms.run(src, dst, n, R, last ? ki.codec : 0, part, s, prof.on ? ev[1] : nullptr);
             if (e != hipSuccess) return hip_fail(e, "merge pass launch");
             if (prof.on) {
-                (void)hipEventRecord(ev[1], s);
-                prof.pending.push_back({PK_MERGE, ev[0], ev[1], n, true});
+                (void)hipEventRecord(ev[2], s);
+                // ev[1] closes the partition and opens the merge; the
+                // partition record owns it
+                prof.pending.push_back({PK_PARTITION, ev[0], ev[1], n, true});
+                prof.pending.push_back({PK_MERGE, ev[1], ev[2], n, false});
             }
             std::swap(src, dst);
         }
@@ -268,17 +289,23 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
     if (tiny_max > small_max) tiny_max = small_max;
     const uint64_t LT = lb.tile(), MT = mb.tile();
     const uint64_t w = (uint64_t)ki.width;
+    const uint64_t plain_min = tune[ki.kc].v[TP_SEG_PLAIN_MIN] > 0 ? (uint64_t)1 << tune[ki.kc].v[TP_SEG_PLAIN_MIN]
+                                                                    : UINT64_MAX;
+    const bool seg_table = tune[ki.kc].v[TP_PLAIN_CORANK] == 0;
     auto at = [w](const void *p, uint64_t i) { return (void *)((char *)p + i * w); };
 
     std::vector<TileDescH> td[3];  // tiny, small, large
     struct Large { size_t i; int passes; };
     std::vector<Large> large;
+    std::vector<size_t> plain;  // segments long enough to run as plain sorts
     for (size_t i = 0; i < nseg; i++) {
         const SegSpec &g = segs[i];
         if (g.len == 0) continue;
         if (!g.in || !g.fin || !g.other || g.fin == g.other)
             return fail(OGS_ERR_ARG, "segment %zu: NULL buffer or fin == other", i);
-        if (g.len <= tiny_max) {
+        if (g.len >= plain_min) {
+            plain.push_back(i);
+        } else if (g.len <= tiny_max) {
             td[0].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
         } else if (g.len <= small_max) {
             td[1].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
@@ -320,11 +347,24 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
     }
     const size_t o_desc = bytes;
     bytes = align(bytes + desc.size() * sizeof(MergeDescH));
+    // plain-class segments first (each a full plain sort on this stream)
+    for (size_t i : plain) {
+        const SegSpec &g = segs[i];
+        void *res = nullptr;
+        int rc = sort(kt, g.in, g.fin, g.other, g.len, &res, s);
+        if (rc) return rc;
+    }
     if (bytes == 0) return OGS_OK;
     Workspace *ws_ = workspace(s);
     std::lock_guard<std::mutex> g(ws_->mu);
     int rc = ws_->reserve(bytes);
     if (rc) return rc;
+    uint64_t max_pass = 0;
+    for (int p = 0; p < max_p; p++) max_pass = std::max<uint64_t>(max_pass, pass_cnt[p]);
+    if (seg_table && max_pass) {
+        rc = ws_->reserve_part(max_pass * sizeof(uint64_t));
+        if (rc) return rc;
+    }
     // the previous upload from this staging buffer must have been consumed
     e = hipEventSynchronize(ws_->staged);
     if (e != hipSuccess) return hip_fail(e, "staging wait");
@@ -343,7 +383,8 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
         if (e != hipSuccess) return hip_fail(e, "batched tile-sort launch");
     }
     for (int p = 0; p < max_p; p++) {
-        e = mb.run_seg(D + o_desc + pass_start[p] * sizeof(MergeDescH), pass_cnt[p], LT << p, ki.codec, s);
+        e = mb.run_seg(D + o_desc + pass_start[p] * sizeof(MergeDescH), pass_cnt[p], LT << p, ki.codec,
+                       seg_table ? (uint64_t *)ws_->part : nullptr, s);
         if (e != hipSuccess) return hip_fail(e, "batched merge launch");
     }
     return OGS_OK;

@@ -26,7 +26,7 @@ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
 
 const char *kParamNames[TP_COUNT] = {"plain_block",     "plain_merge", "seg_small_block",
                                      "seg_small_max",   "seg_large_block", "seg_merge",
-                                     "seg_tiny_block",  "seg_tiny_max", "plain_corank"};
+                                     "seg_tiny_block",  "seg_tiny_max", "plain_corank", "seg_plain_min"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -127,7 +127,7 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
     const int p = param_index(param);
     if (p < 0) return fail(OGS_ERR_ARG, "unknown tuning parameter '%s'", param ? param : "(null)");
     int cnt = 0;
-    const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX;
+    const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX || p == TP_SEG_PLAIN_MIN;
     if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
     else if (p == TP_PLAIN_CORANK) cnt = 2;
     else if (!is_count) block_shapes(ki.kc, &cnt);

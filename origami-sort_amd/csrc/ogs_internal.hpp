@@ -36,8 +36,9 @@ enum TuneParam {
     TP_SEG_MERGE = 5,       // merge shape of segmented passes
     TP_SEG_TINY_BLOCK = 6,  // block-sort shape of the tiny segment class
     TP_SEG_TINY_MAX = 7,    // longest segment in the tiny class (0 = no tiny class)
-    TP_PLAIN_CORANK = 8,    // plain merges: 0 partition kernel + table, 1 co-ranks inside the merge kernel
-    TP_COUNT = 9
+    TP_PLAIN_CORANK = 8,    // merges: 0 partition kernel + table, 1 co-ranks inside the merge kernel
+    TP_SEG_PLAIN_MIN = 9,   // log2 of the shortest batch segment run as a plain sort (0 = never)
+    TP_COUNT = 10
 };
 struct Tuning {
     int v[TP_COUNT];
@@ -48,8 +49,11 @@ struct Tuning {
 // stream is safe; growing it frees the old block (hipFree synchronises).
 struct Workspace {
     std::mutex mu;  // one batched call at a time per stream
-    void *dev = nullptr;
+    void *dev = nullptr;   // descriptor tables
     size_t cap = 0;
+    void *part = nullptr;  // co-rank tables
+    size_t part_cap = 0;
+    int reserve_part(size_t bytes);
     std::vector<uint8_t> host;       // staging for descriptor uploads
     hipEvent_t staged = nullptr;     // recorded after the last upload from `host`
     int reserve(size_t bytes);

@@ -86,8 +86,7 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-__global__ __launch_bounds__(BLOCK) void k_block_sort(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
-                                                      int cin, int cout) {
+__global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out, uint64_t n, int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K sk[TILE];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
@@ -143,20 +142,10 @@ __device__ __forceinline__ PairGeom pair_geom(uint64_t o, uint64_t n, uint64_t R
 // seven dependent steps with 16 lanes).  part[j] = A items among the first
 // j*tile outputs of tile j's pair.
 template <typename K, bool HV, int LANES>
-__global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint64_t R, uint32_t tile,
-                                  uint64_t ntiles, uint64_t *__restrict__ part) {
-    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = (int)(gt % LANES);
-    const int gshift = (int)(threadIdx.x & 63) & ~(LANES - 1);
-    uint64_t j = gt / LANES;
-    const bool live = j < ntiles;
-    if (!live) j = ntiles - 1;
-    const uint64_t o = j * tile;
-    const PairGeom g = pair_geom(o, n, R);
-    const uint64_t diag = o - g.pb;
-    const uint64_t abase = g.pb, bbase = g.pb + g.la;
-    uint64_t lo = diag > g.lb ? diag - g.lb : 0;
-    uint64_t hi = diag < g.la ? diag : g.la;
+__device__ __forceinline__ uint64_t group_corank(const void *in, uint64_t abase, uint64_t la, uint64_t bbase,
+                                                 uint64_t lb, uint64_t diag, int l, int gshift) {
+    uint64_t lo = diag > lb ? diag - lb : 0;
+    uint64_t hi = diag < la ? diag : la;
     while (hi - lo > (uint64_t)LANES) {
         const uint64_t span = hi - lo;
         const uint64_t m = lo + (span * (uint64_t)(l + 1)) / (LANES + 1);
@@ -178,7 +167,21 @@ __global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint6
         p = GMem<K, HV>::load_key(in, bbase + diag - 1 - m) < GMem<K, HV>::load_key(in, abase + m);
     }
     const uint64_t bits = (__ballot(p) >> gshift) & ((1ull << LANES) - 1);
-    const uint64_t a = bits ? lo + (uint64_t)(__ffsll((unsigned long long)bits) - 1) : hi;
+    return bits ? lo + (uint64_t)(__ffsll((unsigned long long)bits) - 1) : hi;
+}
+
+template <typename K, bool HV, int LANES>
+__global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint64_t R, uint32_t tile,
+                                  uint64_t ntiles, uint64_t *__restrict__ part) {
+    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)(gt % LANES);
+    const int gshift = (int)(threadIdx.x & 63) & ~(LANES - 1);
+    uint64_t j = gt / LANES;
+    const bool live = j < ntiles;
+    if (!live) j = ntiles - 1;
+    const uint64_t o = j * tile;
+    const PairGeom g = pair_geom(o, n, R);
+    const uint64_t a = group_corank<K, HV, LANES>(in, g.pb, g.la, g.pb + g.la, g.lb, o - g.pb, l, gshift);
     if (live && l == 0) part[j] = a;
 }
 
@@ -269,8 +272,27 @@ struct MergeDesc {
 static_assert(sizeof(MergeDesc) == 32, "descriptor layout is shared with the host");
 static constexpr uint64_t kLastBit = 1ull << 62;
 
+template <typename K, bool HV, int LANES>
+__global__ void k_seg_merge_partition(const MergeDesc *__restrict__ tiles, uint64_t ntiles, uint64_t R,
+                                      uint64_t *__restrict__ part) {
+    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)(gt % LANES);
+    const int gshift = (int)(threadIdx.x & 63) & ~(LANES - 1);
+    uint64_t t = gt / LANES;
+    const bool live = t < ntiles;
+    if (!live) t = ntiles - 1;
+    const MergeDesc d = tiles[t];
+    const uint64_t o0 = d.o0 & ~kLastBit;
+    const PairGeom g = pair_geom(o0, d.len, R);
+    const uint64_t a = group_corank<K, HV, LANES>(d.src, g.pb, g.la, g.pb + g.la, g.lb, o0 - g.pb, l, gshift);
+    if (live && l == 0) part[t] = a;
+}
+
+// part[t] = co-rank of descriptor t's tile start (the next descriptor of the
+// same segment supplies the end), or nullptr to search inside the kernel.
 template <typename K, bool HV, int BLOCK, int IPT>
-__global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__restrict__ tiles, uint64_t R, int cout) {
+__global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__restrict__ tiles, uint64_t R, int cout,
+                                                          const uint64_t *__restrict__ part) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K sk[TILE];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
@@ -278,7 +300,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__res
     const MergeDesc d = tiles[blockIdx.x];
     const int co = (d.o0 & kLastBit) ? cout : 0;
     const uint64_t o0 = d.o0 & ~kLastBit;
-    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco, nullptr);
+    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco,
+                                  part ? part + blockIdx.x : nullptr);
 }
 
 // ----------------------------------------------------------------------------
@@ -307,7 +330,7 @@ static hipError_t launch_seg_block(const void *tiles, uint64_t ntiles, int cin, 
 // entries) is filled by the partition kernel first.
 template <typename K, bool HV, int BLOCK, int IPT>
 static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part,
-                               hipStream_t s) {
+                               hipStream_t s, hipEvent_t mid) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     constexpr int LANES = 16;
     const uint64_t ntiles = (n + TILE - 1) / TILE;
@@ -317,16 +340,24 @@ static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R
         hipLaunchKernelGGL((k_merge_partition<K, HV, LANES>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
                            s, in, n, R, (uint32_t)TILE, ntiles, part);
     }
+    if (mid) (void)hipEventRecord(mid, s);
     hipLaunchKernelGGL((k_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n, R,
                        cout, (const uint64_t *)part);
     return hipGetLastError();
 }
 
 template <typename K, bool HV, int BLOCK, int IPT>
-static hipError_t launch_seg_merge(const void *tiles, uint64_t ntiles, uint64_t R, int cout, hipStream_t s) {
+static hipError_t launch_seg_merge(const void *tiles, uint64_t ntiles, uint64_t R, int cout, uint64_t *part,
+                                   hipStream_t s) {
+    constexpr int LANES = 16;
     if (ntiles == 0) return hipSuccess;
+    if (part) {
+        const uint64_t threads = ntiles * LANES;
+        hipLaunchKernelGGL((k_seg_merge_partition<K, HV, LANES>), dim3((unsigned)((threads + 255) / 256)), dim3(256),
+                           0, s, (const MergeDesc *)tiles, ntiles, R, part);
+    }
     hipLaunchKernelGGL((k_seg_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s,
-                       (const MergeDesc *)tiles, R, cout);
+                       (const MergeDesc *)tiles, R, cout, (const uint64_t *)part);
     return hipGetLastError();
 }
 

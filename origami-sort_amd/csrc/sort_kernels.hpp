@@ -33,9 +33,13 @@ typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin
 typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s);
 // part == nullptr: co-ranks found inside the merge kernel; else a table of
 // ceil(n / tile) entries that the partition kernel fills first.
+// `mid` (nullable) is recorded between the partition and the merge kernel.
 typedef hipError_t (*MergeLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part,
-                                  hipStream_t s);
-typedef hipError_t (*SegMergeLaunch)(const void *tiles, uint64_t ntiles, uint64_t R, int cout, hipStream_t s);
+                                  hipStream_t s, hipEvent_t mid);
+// part: ntiles entries for the descriptors' start co-ranks (filled by a
+// partition kernel first), or nullptr to search inside the merge kernel.
+typedef hipError_t (*SegMergeLaunch)(const void *tiles, uint64_t ntiles, uint64_t R, int cout, uint64_t *part,
+                                     hipStream_t s);
 
 struct BlockShapeFns {
     int block, ipt, wlev;

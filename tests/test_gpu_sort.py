@@ -202,8 +202,9 @@ def test_every_compiled_shape(ctx, kind):
         ctx.tune_set(kind, "plain_corank", 0)
 
 
+@pytest.mark.parametrize("plain_min", [22, 16])
 @pytest.mark.parametrize("kind", [ogs.U32, ogs.KV_I64I64, ogs.F32])
-def test_pointer_batch(ctx, kind):
+def test_pointer_batch(ctx, kind, plain_min):
     # ogs_sort_batch: independent arrays, out-of-place, in-place (out == in) and
     # without scratch (in clobbered), all in one launch sequence
     import torch
@@ -226,8 +227,12 @@ def test_pointer_batch(ctx, kind):
             out, scr = torch.empty_like(din), None
             items.append((din.data_ptr(), out.data_ptr(), 0, n))
         keep.append((din, out, scr))
-    ctx.sort_batch_ptrs(kind, items)
-    torch.cuda.synchronize()
+    ctx.tune_set(kind, "seg_plain_min", plain_min)  # 16: the largest arrays run as plain sorts
+    try:
+        ctx.sort_batch_ptrs(kind, items)
+        torch.cuda.synchronize()
+    finally:
+        ctx.tune_set(kind, "seg_plain_min", 22)
     for a, (din, out, scr), n in zip(arrays, keep, lens):
         if n == 0:
             continue
