@@ -184,6 +184,7 @@ typedef struct {
     int streams;                 /* batches in flight at once (1..16) */
     uint32_t max_batch_requests; /* 0 = 1024 */
     uint64_t max_batch_items;    /* 0 = unlimited */
+    uint64_t coalesce_max_items; /* longer requests run as batches of one; 0 = 2^20 */
 } ogs_server_config;
 typedef struct {
     uint64_t submitted, completed, rejected, batches, batched_requests, items;

@@ -127,6 +127,7 @@ struct Server {
     bool sched_done = false;   // the scheduler has issued its last batch
     uint64_t releases = 0;     // arena releases so far (admission retries on change)
     uint64_t next_ticket = 1;
+    uint64_t coalesce_max = 1u << 20;  // longest request that is batched with others
     double t0 = 0;
     ogs_server_stats stats{};
 
@@ -171,9 +172,13 @@ void Server::schedule_loop() {
         size_t bytes = 0;
         uint64_t items = 0;
         size_t scanned = 0;
+        // long requests run alone (a batch finishes when its last request
+        // does, so coalescing them would only delay the short ones)
+        const bool alone = head->n > coalesce_max;
         for (auto it = pending.begin(); it != pending.end() && scanned < 256; ++it, ++scanned) {
             Request *r = *it;
-            if (r->kt != head->kt) continue;
+            if (r->kt != head->kt || (r != head && (alone || r->n > coalesce_max))) continue;
+            if (alone && !take.empty()) break;
             const size_t f = footprint(r);
             if (!take.empty() && (take.size() >= cfg.max_batch_requests || items + r->n > cfg.max_batch_items ||
                                   bytes + f > arena.capacity()))
@@ -313,6 +318,7 @@ int ogs_server_create(ogs_server **srv, ogs_ctx *ctx, const ogs_server_config *c
     s->cfg = *cfg;
     if (!s->cfg.max_batch_requests) s->cfg.max_batch_requests = 1024;
     if (!s->cfg.max_batch_items) s->cfg.max_batch_items = UINT64_MAX;
+    if (cfg->coalesce_max_items) s->coalesce_max = cfg->coalesce_max_items;
     hipError_t e = hipSetDevice(s->ctx->device);
     if (e == hipSuccess) e = hipMalloc(&s->arena_base, cfg->hbm_budget_bytes);
     if (e != hipSuccess) {

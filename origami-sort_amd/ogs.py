@@ -107,7 +107,8 @@ def shapes(kind, merge):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("streams", ctypes.c_int),
-                ("max_batch_requests", ctypes.c_uint32), ("max_batch_items", ctypes.c_uint64)]
+                ("max_batch_requests", ctypes.c_uint32), ("max_batch_items", ctypes.c_uint64),
+                ("coalesce_max_items", ctypes.c_uint64)]
 
 
 class ServerStats(ctypes.Structure):
@@ -121,10 +122,11 @@ SUBMIT_HOST = 1
 class Server:
     """ogs_server: continuous-batching sort server on a context's device."""
 
-    def __init__(self, ctx, hbm_budget_bytes, streams=4, max_batch_requests=0, max_batch_items=0):
+    def __init__(self, ctx, hbm_budget_bytes, streams=4, max_batch_requests=0, max_batch_items=0,
+                 coalesce_max_items=0):
         self.ctx = ctx
         self.h = ctypes.c_void_p()
-        cfg = ServerConfig(hbm_budget_bytes, streams, max_batch_requests, max_batch_items)
+        cfg = ServerConfig(hbm_budget_bytes, streams, max_batch_requests, max_batch_items, coalesce_max_items)
         check(lib().ogs_server_create(ctypes.byref(self.h), ctx.h, ctypes.byref(cfg)))
 
     def submit(self, kind, src, dst, n, host=False):
