@@ -127,7 +127,7 @@ def bench_batched(ctx, total_log2, max_seg_log2, seed=4):
             "unit": "G keys/s", "speedup": round(t_p / t_b, 2), "identical_results": bool(same and ok_sorted)}
 
 
-def bench_serving(ctx, nreq, rate_factor, seed=5):
+def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5):
     """Serving loop: Poisson arrivals of ORDER-BY requests (key type uniform over
     u32/u64/u64+u64, length log-uniform in [2^10, 2^26], keys uniform or Zipf 50/50,
     device-resident), served by ogs_server (continuous batching, 4 streams) and,
@@ -187,7 +187,7 @@ def bench_serving(ctx, nreq, rate_factor, seed=5):
     # the server at the same arrival times
     dsts = [torch.empty_like(src) for _, _, src in reqs]
     budget = 3 * max(n * ogs.WIDTH[k] for k, n, _ in reqs) + (256 << 20)
-    srv = ogs.Server(ctx, hbm_budget_bytes=int(budget), streams=4)
+    srv = ogs.Server(ctx, hbm_budget_bytes=int(budget), streams=streams)
     torch.cuda.synchronize()
     tickets = []
     t0 = time.perf_counter()
@@ -214,7 +214,7 @@ def bench_serving(ctx, nreq, rate_factor, seed=5):
                        "p99_ms": round(pct(lat_v, 99), 3), "batches": st["batches"]},
             "serial_plain": {"keys_per_s": round(total_keys / span_s / 1e9, 4), "p50_ms": round(pct(lat_s, 50), 3),
                              "p99_ms": round(pct(lat_s, 99), 3)},
-            "unit": "G keys/s", "identical_results": bool(same),
+            "streams": streams, "unit": "G keys/s", "identical_results": bool(same),
             "note": "device-resident requests; latency = submit (arrival) to result ready"}
 
 
@@ -230,6 +230,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--batched-log2-total", type=int, default=30)
     ap.add_argument("--serving-requests", type=int, default=48)
+    ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--serving-load", type=float, default=1.5,
                     help="offered request rate as a multiple of the serial plain-launch capacity")
     ap.add_argument("--copy-calib", action="store_true",
@@ -336,7 +337,7 @@ def main():
         line["batched"] = bench_batched(ctx, args.batched_log2_total, 20)
     if not args.no_serving:
         torch.cuda.empty_cache()
-        line["serving"] = bench_serving(ctx, args.serving_requests, args.serving_load)
+        line["serving"] = bench_serving(ctx, args.serving_requests, args.serving_load, args.serving_streams)
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
         import oracle

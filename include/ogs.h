@@ -139,7 +139,8 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
  * "seg_large_block", "seg_merge", "plain_corank" (0: a partition kernel
  * computes every merge tile's co-rank first, 1: each merge workgroup searches
  * its own), "seg_plain_min" (log2 of the shortest batch segment that runs as
- * a plain sort instead; 0 = never).  Block and merge parameters take an index
+ * a plain sort instead; 0 = never), "partition_lanes" (lanes per merge tile
+ * in the partition kernel: 1, 4, 8 or 16).  Block and merge parameters take an index
  * into ogs_shape_info's lists; *_max take an item count.  Segments of up to
  * seg_tiny_max items (0 = no tiny class) get one workgroup of the tiny shape
  * each, segments up to seg_small_max (0 = the small shape's tile) one

@@ -67,7 +67,7 @@ def tune_class(ctx, name, kind, quick):
     reps = 3 if quick else 5
     result = {}
     # ---- plain sort
-    n = (1 << 26) // (w // 4) if not quick else (1 << 24) // (w // 4)
+    n = (1 << 28) // (w // 4) if not quick else (1 << 24) // (w // 4)
     host = ogs.generate(kind, ogs.UNIFORM, n, seed=42)
     d_in = ogs.to_device(host)
     a = torch.empty_like(d_in)
@@ -75,7 +75,7 @@ def tune_class(ctx, name, kind, quick):
     best = None
     for bi, (bb, bp, bw) in enumerate(bs):
         for mi, (mb, mp, _) in enumerate(ms):
-            if mb * mp > 2 * bb * bp or bb * bp < 2048 or mb * mp < 2048:
+            if mb * mp > 2 * bb * bp or bb * bp < 2048 or mb * mp < 4096:
                 continue
             for corank in (0, 1):
                 ctx.tune_set(kind, "plain_block", bi)
@@ -89,6 +89,15 @@ def tune_class(ctx, name, kind, quick):
     ctx.tune_set(kind, "plain_merge", best[2])
     ctx.tune_set(kind, "plain_corank", best[3])
     result["plain_block"], result["plain_merge"], result["plain_corank"] = best[1], best[2], best[3]
+    if best[3] == 0:
+        for lanes in (1, 4, 8, 16):
+            ctx.tune_set(kind, "partition_lanes", lanes)
+            t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
+            log(f"  {name} plain partition_lanes={lanes}: {t * 1e3:.3f} ms")
+            if t < best[0]:
+                best = (t, best[1], best[2], best[3])
+                result["partition_lanes"] = lanes
+        ctx.tune_set(kind, "partition_lanes", result.get("partition_lanes", 4))
     result["_plain"] = {"n": n, "ms": best[0] * 1e3, "gkeys": n / best[0] / 1e9}
     del d_in, a, b
     # ---- segmented batch: log-uniform lengths in [1, 2^20]

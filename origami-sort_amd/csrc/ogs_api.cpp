@@ -66,6 +66,7 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_SEG_TINY_MAX] = 0;
     t.v[TP_PLAIN_CORANK] = 0;
     t.v[TP_SEG_PLAIN_MIN] = 22;
+    t.v[TP_PARTITION_LANES] = 4;
     return t;
 }
 
@@ -237,7 +238,8 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
                 for (int k = 0; k < 3; k++) ev[k] = prof.get();
                 (void)hipEventRecord(ev[0], s);
             }
-            e = ms.run(src, dst, n, R, last ? ki.codec : 0, part, s, prof.on ? ev[1] : nullptr);
+            e = ms.run(src, dst, n, R, last ? ki.codec : 0, part, tune[ki.kc].v[TP_PARTITION_LANES], s,
+                       prof.on ? ev[1] : nullptr);
             if (e != hipSuccess) return hip_fail(e, "merge pass launch");
             if (prof.on) {
                 (void)hipEventRecord(ev[2], s);
@@ -384,7 +386,7 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
     }
     for (int p = 0; p < max_p; p++) {
         e = mb.run_seg(D + o_desc + pass_start[p] * sizeof(MergeDescH), pass_cnt[p], LT << p, ki.codec,
-                       seg_table ? (uint64_t *)ws_->part : nullptr, s);
+                       seg_table ? (uint64_t *)ws_->part : nullptr, tune[ki.kc].v[TP_PARTITION_LANES], s);
         if (e != hipSuccess) return hip_fail(e, "batched merge launch");
     }
     return OGS_OK;

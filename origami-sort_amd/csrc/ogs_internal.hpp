@@ -38,7 +38,8 @@ enum TuneParam {
     TP_SEG_TINY_MAX = 7,    // longest segment in the tiny class (0 = no tiny class)
     TP_PLAIN_CORANK = 8,    // merges: 0 partition kernel + table, 1 co-ranks inside the merge kernel
     TP_SEG_PLAIN_MIN = 9,   // log2 of the shortest batch segment run as a plain sort (0 = never)
-    TP_COUNT = 10
+    TP_PARTITION_LANES = 10,// lanes per merge tile in the partition kernel: 1, 4, 8 or 16
+    TP_COUNT = 11
 };
 struct Tuning {
     int v[TP_COUNT];

@@ -328,18 +328,49 @@ static hipError_t launch_seg_block(const void *tiles, uint64_t ntiles, int cin, 
 
 // part == nullptr: co-ranks found inside the merge kernel; else `part` (ntiles
 // entries) is filled by the partition kernel first.
+// Partition launch with `lanes` lanes per tile (1, 4, 8 or 16): fewer lanes
+// issue fewer loads, more lanes take fewer dependent steps.
+template <typename K, bool HV, int LANES>
+static void launch_partition_l(const void *in, uint64_t n, uint64_t R, uint32_t tile, uint64_t ntiles,
+                               uint64_t *part, hipStream_t s) {
+    const uint64_t threads = ntiles * LANES;
+    hipLaunchKernelGGL((k_merge_partition<K, HV, LANES>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                       in, n, R, tile, ntiles, part);
+}
+template <typename K, bool HV>
+static void launch_partition(int lanes, const void *in, uint64_t n, uint64_t R, uint32_t tile, uint64_t ntiles,
+                             uint64_t *part, hipStream_t s) {
+    switch (lanes) {
+    case 1: launch_partition_l<K, HV, 1>(in, n, R, tile, ntiles, part, s); break;
+    case 4: launch_partition_l<K, HV, 4>(in, n, R, tile, ntiles, part, s); break;
+    case 8: launch_partition_l<K, HV, 8>(in, n, R, tile, ntiles, part, s); break;
+    default: launch_partition_l<K, HV, 16>(in, n, R, tile, ntiles, part, s); break;
+    }
+}
+template <typename K, bool HV, int LANES>
+static void launch_seg_partition_l(const void *tiles, uint64_t ntiles, uint64_t R, uint64_t *part, hipStream_t s) {
+    const uint64_t threads = ntiles * LANES;
+    hipLaunchKernelGGL((k_seg_merge_partition<K, HV, LANES>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       s, (const MergeDesc *)tiles, ntiles, R, part);
+}
+template <typename K, bool HV>
+static void launch_seg_partition(int lanes, const void *tiles, uint64_t ntiles, uint64_t R, uint64_t *part,
+                                 hipStream_t s) {
+    switch (lanes) {
+    case 1: launch_seg_partition_l<K, HV, 1>(tiles, ntiles, R, part, s); break;
+    case 4: launch_seg_partition_l<K, HV, 4>(tiles, ntiles, R, part, s); break;
+    case 8: launch_seg_partition_l<K, HV, 8>(tiles, ntiles, R, part, s); break;
+    default: launch_seg_partition_l<K, HV, 16>(tiles, ntiles, R, part, s); break;
+    }
+}
+
 template <typename K, bool HV, int BLOCK, int IPT>
-static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part,
+static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part, int lanes,
                                hipStream_t s, hipEvent_t mid) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
-    constexpr int LANES = 16;
     const uint64_t ntiles = (n + TILE - 1) / TILE;
     if (ntiles == 0) return hipSuccess;
-    if (part) {
-        const uint64_t threads = ntiles * LANES;
-        hipLaunchKernelGGL((k_merge_partition<K, HV, LANES>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
-                           s, in, n, R, (uint32_t)TILE, ntiles, part);
-    }
+    if (part) launch_partition<K, HV>(lanes, in, n, R, (uint32_t)TILE, ntiles, part, s);
     if (mid) (void)hipEventRecord(mid, s);
     hipLaunchKernelGGL((k_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n, R,
                        cout, (const uint64_t *)part);
@@ -348,14 +379,9 @@ static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R
 
 template <typename K, bool HV, int BLOCK, int IPT>
 static hipError_t launch_seg_merge(const void *tiles, uint64_t ntiles, uint64_t R, int cout, uint64_t *part,
-                                   hipStream_t s) {
-    constexpr int LANES = 16;
+                                   int lanes, hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
-    if (part) {
-        const uint64_t threads = ntiles * LANES;
-        hipLaunchKernelGGL((k_seg_merge_partition<K, HV, LANES>), dim3((unsigned)((threads + 255) / 256)), dim3(256),
-                           0, s, (const MergeDesc *)tiles, ntiles, R, part);
-    }
+    if (part) launch_seg_partition<K, HV>(lanes, tiles, ntiles, R, part, s);
     hipLaunchKernelGGL((k_seg_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s,
                        (const MergeDesc *)tiles, R, cout, (const uint64_t *)part);
     return hipGetLastError();

@@ -34,12 +34,13 @@ typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin
 // part == nullptr: co-ranks found inside the merge kernel; else a table of
 // ceil(n / tile) entries that the partition kernel fills first.
 // `mid` (nullable) is recorded between the partition and the merge kernel.
+// `lanes` = lanes per tile of the partition kernel (1, 4, 8, 16).
 typedef hipError_t (*MergeLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part,
-                                  hipStream_t s, hipEvent_t mid);
+                                  int lanes, hipStream_t s, hipEvent_t mid);
 // part: ntiles entries for the descriptors' start co-ranks (filled by a
 // partition kernel first), or nullptr to search inside the merge kernel.
 typedef hipError_t (*SegMergeLaunch)(const void *tiles, uint64_t ntiles, uint64_t R, int cout, uint64_t *part,
-                                     hipStream_t s);
+                                     int lanes, hipStream_t s);
 
 struct BlockShapeFns {
     int block, ipt, wlev;

@@ -240,7 +240,8 @@ class Context:
         return a
 
     TUNE_PARAMS = ("plain_block", "plain_merge", "plain_corank", "seg_tiny_block", "seg_tiny_max",
-                   "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge", "seg_plain_min")
+                   "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge", "seg_plain_min",
+                   "partition_lanes")
 
     def pass_count(self, kind, n):
         return check(lib().ogs_pass_count(self.h, kind, n))
