@@ -292,7 +292,7 @@ def main():
     value = n / (ms_step / 1e3) / 1e9
 
     kernels = {}
-    for kid, name in ((0, "block_sort"), (1, "merge_partition"), (2, "merge_pass")):
+    for kid, name in ((0, "block_sort"), (1, "merge_partition"), (2, "merge_pass"), (3, "merge4_pass")):
         ms, cnt, items = ctx.profile_read(kid)
         if cnt:
             kernels[name] = {"total_ms": ms, "launches": cnt, "mean_ms": ms / cnt, "items": items}

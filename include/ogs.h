@@ -140,7 +140,9 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
  * computes every merge tile's co-rank first, 1: each merge workgroup searches
  * its own), "seg_plain_min" (log2 of the shortest batch segment that runs as
  * a plain sort instead; 0 = never), "partition_lanes" (lanes per merge tile
- * in the partition kernel: 1, 4, 8 or 16).  Block and merge parameters take an index
+ * in the partition kernel: 1, 4, 8 or 16), "plain_fanin" (2: one merge level per
+ * HBM pass; 4: two levels per pass, the last pass 2-way when one level is
+ * left).  Block and merge parameters take an index
  * into ogs_shape_info's lists; *_max take an item count.  Segments of up to
  * seg_tiny_max items (0 = no tiny class) get one workgroup of the tiny shape
  * each, segments up to seg_small_max (0 = the small shape's tile) one
@@ -161,8 +163,8 @@ int ogs_tune_load(ogs_ctx *ctx, const char *path);
 int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *wlev);
 
 /* Per-kernel timing with HIP events recorded on the launching stream, for
- * plain sorts issued while enabled.  kind: 0 block sort, 1 merge partition,
- * 2 merge pass.  read returns the summed milliseconds, the launch count and
+ * plain sorts issued while enabled.  kind: 0 block sort, 1 merge partition
+ * (co-rank tables), 2 2-way merge pass, 3 4-way merge pass.  read returns the summed milliseconds, the launch count and
  * the items those launches covered (waits for the recorded events). */
 int ogs_profile_enable(ogs_ctx *ctx, int on);
 int ogs_profile_read(ogs_ctx *ctx, int kind, double *total_ms, uint64_t *launches, uint64_t *items);

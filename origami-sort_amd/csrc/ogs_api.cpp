@@ -67,6 +67,7 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_PLAIN_CORANK] = 0;
     t.v[TP_SEG_PLAIN_MIN] = 22;
     t.v[TP_PARTITION_LANES] = 4;
+    t.v[TP_PLAIN_FANIN] = 2;
     return t;
 }
 
@@ -200,10 +201,18 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     const MergeShapeFns &ms = merge_shape(ki.kc, TP_PLAIN_MERGE, bs.tile());
     const uint64_t T = bs.tile();
     const bool one = n <= T;
-    // passes after the block sort; start on the buffer that makes the last
-    // pass land in `data` when reading from a separate input
-    int merges = 0;
-    for (uint64_t R = T; R < n; R *= 2) merges++;
+    // merge passes after the block sort: 4-way passes (runs R -> 4R, two
+    // levels of the merge tree per HBM pass) while at least two levels remain
+    // and the table asks for fan-in 4, else 2-way; start on the buffer that
+    // makes the last pass land in `data` when reading from a separate input
+    const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4;
+    std::vector<int> ways;
+    for (uint64_t R = T; R < n;) {
+        const int wy = (quad && 2 * R < n) ? 4 : 2;
+        ways.push_back(wy);
+        R *= (uint64_t)wy;
+    }
+    const int merges = (int)ways.size();
     void *src = scratch, *dst = data;
     const void *bin = data;
     if (in) {
@@ -223,30 +232,44 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
         (void)hipEventRecord(ev[1], s);
         prof.pending.push_back({PK_BLOCK, ev[0], ev[1], n, true});
     }
-    uint64_t *part = nullptr;
-    if (!one && tune[ki.kc].v[TP_PLAIN_CORANK] == 0) {
+    // co-rank tables: 2-way tile co-ranks, or for 4-way passes the sample
+    // table Q (every S outputs) followed by the per-tile split table
+    const uint64_t MT = ms.tile();
+    const uint32_t S = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(2048, MT), 2 * T);
+    uint64_t *part = nullptr, *qtab = nullptr;
+    const bool table = tune[ki.kc].v[TP_PLAIN_CORANK] == 0;
+    if (!one && (table || quad)) {
         Workspace *w = workspace(s);
         std::lock_guard<std::mutex> g(w->mu);
-        int rc = w->reserve_part(((n + ms.tile() - 1) / ms.tile()) * sizeof(uint64_t));
+        const uint64_t ntile = (n + MT - 1) / MT, nq = (n + S - 1) / S;
+        int rc = w->reserve_part((std::max(ntile, nq) + 3 * ntile + 8) * sizeof(uint64_t));
         if (rc) return rc;
         part = (uint64_t *)w->part;
+        qtab = part + std::max(ntile, nq) + 4;
     }
     if (!one) {
-        for (uint64_t R = T; R < n; R *= 2) {
-            const bool last = 2 * R >= n;
+        uint64_t R = T;
+        for (int pi = 0; pi < merges; pi++) {
+            const int wy = ways[pi];
+            const bool last = pi == merges - 1;
             if (prof.on) {
                 for (int k = 0; k < 3; k++) ev[k] = prof.get();
                 (void)hipEventRecord(ev[0], s);
             }
-            e = ms.run(src, dst, n, R, last ? ki.codec : 0, part, tune[ki.kc].v[TP_PARTITION_LANES], s,
-                       prof.on ? ev[1] : nullptr);
+            if (wy == 4)
+                e = ms.run_quad(src, dst, n, R, last ? ki.codec : 0, part, qtab, S, tune[ki.kc].v[TP_PARTITION_LANES],
+                                s, prof.on ? ev[1] : nullptr);
+            else
+                e = ms.run(src, dst, n, R, last ? ki.codec : 0, table ? part : nullptr,
+                           tune[ki.kc].v[TP_PARTITION_LANES], s, prof.on ? ev[1] : nullptr);
+            R *= (uint64_t)wy;
             if (e != hipSuccess) return hip_fail(e, "merge pass launch");
             if (prof.on) {
                 (void)hipEventRecord(ev[2], s);
                 // ev[1] closes the partition and opens the merge; the
                 // partition record owns it
                 prof.pending.push_back({PK_PARTITION, ev[0], ev[1], n, true});
-                prof.pending.push_back({PK_MERGE, ev[1], ev[2], n, false});
+                prof.pending.push_back({wy == 4 ? PK_MERGE4 : PK_MERGE, ev[1], ev[2], n, false});
             }
             std::swap(src, dst);
         }
@@ -260,8 +283,9 @@ int Ctx::pass_count(ogs_key_t kt, uint64_t n) {
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
     if (n == 0) return 0;
     const uint64_t T = block_shape(ki.kc, TP_PLAIN_BLOCK).tile();
+    const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4;
     int p = 1;
-    for (uint64_t R = T; R < n; R *= 2) p++;
+    for (uint64_t R = T; R < n; p++) R *= (quad && 2 * R < n) ? 4 : 2;
     return p;
 }
 

@@ -27,7 +27,7 @@ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
 const char *kParamNames[TP_COUNT] = {"plain_block",     "plain_merge", "seg_small_block",
                                      "seg_small_max",   "seg_large_block", "seg_merge",
                                      "seg_tiny_block",  "seg_tiny_max", "plain_corank", "seg_plain_min",
-                                     "partition_lanes"};
+                                     "partition_lanes", "plain_fanin"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -131,6 +131,11 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
     const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX || p == TP_SEG_PLAIN_MIN;
     if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
     else if (p == TP_PLAIN_CORANK) cnt = 2;
+    else if (p == TP_PLAIN_FANIN) {
+        if (value != 2 && value != 4) return fail(OGS_ERR_ARG, "plain_fanin must be 2 or 4 (got %d)", value);
+        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
+        return OGS_OK;
+    }
     else if (p == TP_PARTITION_LANES) {
         if (value != 1 && value != 4 && value != 8 && value != 16)
             return fail(OGS_ERR_ARG, "partition_lanes must be 1, 4, 8 or 16 (got %d)", value);

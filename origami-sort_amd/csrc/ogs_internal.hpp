@@ -39,7 +39,8 @@ enum TuneParam {
     TP_PLAIN_CORANK = 8,    // merges: 0 partition kernel + table, 1 co-ranks inside the merge kernel
     TP_SEG_PLAIN_MIN = 9,   // log2 of the shortest batch segment run as a plain sort (0 = never)
     TP_PARTITION_LANES = 10,// lanes per merge tile in the partition kernel: 1, 4, 8 or 16
-    TP_COUNT = 11
+    TP_PLAIN_FANIN = 11,    // plain merges: 2 (one level per HBM pass) or 4 (two levels)
+    TP_COUNT = 12
 };
 struct Tuning {
     int v[TP_COUNT];
@@ -72,8 +73,10 @@ struct SegSpec {
 };
 
 // Per-kernel timing with HIP events recorded on the launching stream
-// (ogs_profile_*): kernel kinds 0 block sort, 1 merge partition, 2 merge pass.
-enum ProfKind { PK_BLOCK = 0, PK_PARTITION = 1, PK_MERGE = 2, PK_COUNT = 3 };
+// (ogs_profile_*): kernel kinds 0 block sort, 1 merge partition (both the
+// 2-way table and the 4-way sample + split tables), 2 2-way merge pass,
+// 3 4-way merge pass.
+enum ProfKind { PK_BLOCK = 0, PK_PARTITION = 1, PK_MERGE = 2, PK_MERGE4 = 3, PK_COUNT = 4 };
 struct Profiler {
     bool on = false;
     struct Rec {
@@ -84,9 +87,9 @@ struct Profiler {
     };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
-    double total_ms[PK_COUNT] = {0, 0, 0};
-    uint64_t count[PK_COUNT] = {0, 0, 0};
-    uint64_t items[PK_COUNT] = {0, 0, 0};
+    double total_ms[PK_COUNT] = {0, 0, 0, 0};
+    uint64_t count[PK_COUNT] = {0, 0, 0, 0};
+    uint64_t items[PK_COUNT] = {0, 0, 0, 0};
     hipEvent_t get();
     int flush();  // waits for pending events and accumulates
     void reset();

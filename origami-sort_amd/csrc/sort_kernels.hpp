@@ -48,10 +48,17 @@ struct BlockShapeFns {
     SegBlockLaunch run_seg;
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
+// One 4-way pass (runs of R -> 4R): the 2-way co-rank sample table Q
+// (ceil(n/S) entries, S | 2R and S | tile), the per-tile split table tab
+// (3*ceil(n/tile) entries), then the merge; `mid` is recorded before the merge.
+typedef hipError_t (*QuadLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *Q,
+                                 uint64_t *tab, uint32_t S, int lanes, hipStream_t s, hipEvent_t mid);
+
 struct MergeShapeFns {
     int block, ipt;
     MergeLaunch run;
     SegMergeLaunch run_seg;
+    QuadLaunch run_quad;
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
 

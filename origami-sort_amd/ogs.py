@@ -211,7 +211,8 @@ class Context:
         check(lib().ogs_profile_reset(self.h))
 
     def profile_read(self, kind):
-        """kind 0 block sort, 1 merge partition, 2 merge pass -> (ms, launches, items)."""
+        """kind 0 block sort, 1 merge partition, 2 2-way merge pass, 3 4-way merge pass
+        -> (ms, launches, items)."""
         ms, cnt, items = ctypes.c_double(), ctypes.c_uint64(), ctypes.c_uint64()
         check(lib().ogs_profile_read(self.h, kind, ctypes.byref(ms), ctypes.byref(cnt), ctypes.byref(items)))
         return ms.value, cnt.value, items.value
@@ -241,7 +242,7 @@ class Context:
 
     TUNE_PARAMS = ("plain_block", "plain_merge", "plain_corank", "seg_tiny_block", "seg_tiny_max",
                    "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge", "seg_plain_min",
-                   "partition_lanes")
+                   "partition_lanes", "plain_fanin")
 
     def pass_count(self, kind, n):
         return check(lib().ogs_pass_count(self.h, kind, n))

@@ -264,3 +264,44 @@ def test_segmented_tiny_class(ctx):
         assert np.array_equal(got, ref)
     finally:
         ctx.tune_set(kind, "seg_tiny_max", 0)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32, ogs.I64])
+@pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.DISTINCT16, ogs.SORTED, ogs.REVERSE, ogs.ZIPF])
+def test_four_way_passes(ctx, kind, dist):
+    # plain_fanin 4: two merge levels per HBM pass (sample table + exact split
+    # search + two LDS merge levels); odd level counts end with a 2-way pass
+    bshapes = ogs.shapes(kind, 0)
+    old = {p: ctx.tune_get(kind, p) for p in ("plain_fanin", "plain_block", "plain_merge")}
+    try:
+        ctx.tune_set(kind, "plain_fanin", 4)
+        for bi in (0, len(bshapes) - 1):
+            ctx.tune_set(kind, "plain_block", bi)
+            tile = bshapes[bi][0] * bshapes[bi][1]
+            for n in (2 * tile + 1, 3 * tile, 4 * tile - 1, 5 * tile + 7, 16 * tile + 3, 33 * tile + 5,
+                      (1 << 20) + 5):
+                say(f"case 4-way kind={ogs.KIND_NAMES[kind]} dist={dist} block={bshapes[bi]} n={n}")
+                d = ogs.NORMAL if kind == ogs.F32 and dist == ogs.UNIFORM else dist
+                a = ogs.generate(kind, d, n, seed=n + dist)
+                assert_same(kind, gpu_sort(ctx, kind, a), a)
+    finally:
+        for p, v in old.items():
+            ctx.tune_set(kind, p, v)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64])
+def test_four_way_every_merge_shape(ctx, kind):
+    mshapes = ogs.shapes(kind, 1)
+    old = {p: ctx.tune_get(kind, p) for p in ("plain_fanin", "plain_block", "plain_merge")}
+    try:
+        ctx.tune_set(kind, "plain_fanin", 4)
+        ctx.tune_set(kind, "plain_block", 0)
+        for mi, ms in enumerate(mshapes):
+            ctx.tune_set(kind, "plain_merge", mi)
+            for n in (300007, 1 << 21):
+                say(f"case 4-way merge shape kind={ogs.KIND_NAMES[kind]} merge={ms} n={n}")
+                a = ogs.generate(kind, ogs.ZIPF if n % 2 else ogs.UNIFORM, n, seed=mi)
+                assert_same(kind, gpu_sort(ctx, kind, a), a)
+    finally:
+        for p, v in old.items():
+            ctx.tune_set(kind, p, v)
