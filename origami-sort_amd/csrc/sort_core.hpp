@@ -262,6 +262,83 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
     }
 }
 
+// Merges outputs [d, d + c) (c <= IPT) of merge(A, B) into registers with CH
+// independent chains: chain k starts on its own diagonal d + k*IPT/CH (its own
+// co-rank search), and the chains advance in lockstep, so each step issues CH
+// independent LDS reads instead of one dependent read per output.
+#ifndef OGS_MERGE_CHAINS
+#define OGS_MERGE_CHAINS 2
+#endif
+template <int IPT, int CH, bool HV, typename K>
+__device__ __forceinline__ void lds_merge_chains(const K *sk, const uint64_t *sv, int a_off, int la, int b_off,
+                                                 int lb, int d, int c, int limit, K (&r)[IPT],
+                                                 uint64_t (&v)[IPT]) {
+    static_assert(IPT % CH == 0, "chains must split IPT evenly");
+    constexpr int L = IPT / CH;
+    const int tot = la + lb;
+    int lo[CH], hi[CH], dg[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        dg[k] = min(d + k * L, tot);
+        lo[k] = dg[k] > lb ? dg[k] - lb : 0;
+        hi[k] = dg[k] < la ? dg[k] : la;
+    }
+    bool busy = true;
+    while (busy) {
+        busy = false;
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            if (lo[k] < hi[k]) {
+                const int mid = (lo[k] + hi[k]) >> 1;
+                const K a = sk[lsw<K>(a_off + mid)];
+                const K b = sk[lsw<K>(b_off + dg[k] - 1 - mid)];
+                if (!(b < a)) lo[k] = mid + 1;
+                else hi[k] = mid;
+                busy |= lo[k] < hi[k];
+            }
+        }
+    }
+    const int ea = a_off + la, eb = b_off + lb;
+    int ia[CH], ib[CH];
+    K ka[CH], kb[CH];
+    uint64_t va[CH], vb[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        ia[k] = a_off + lo[k];
+        ib[k] = b_off + dg[k] - lo[k];
+        ka[k] = sk[lsw<K>(min(ia[k], limit))];
+        kb[k] = sk[lsw<K>(min(ib[k], limit))];
+        va[k] = vb[k] = 0;
+        if constexpr (HV) {
+            va[k] = sv[lsw<uint64_t>(min(ia[k], limit))];
+            vb[k] = sv[lsw<uint64_t>(min(ib[k], limit))];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < L; i++) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            const int o = k * L + i;
+            if (o < c) {
+                const bool takeA = ib[k] >= eb || (ia[k] < ea && !(kb[k] < ka[k]));
+                r[o] = takeA ? ka[k] : kb[k];
+                if constexpr (HV) v[o] = takeA ? va[k] : vb[k];
+                ia[k] += takeA;
+                ib[k] += !takeA;
+                const int nx = min(takeA ? ia[k] : ib[k], limit);
+                const K nk = sk[lsw<K>(nx)];
+                ka[k] = takeA ? nk : ka[k];
+                kb[k] = takeA ? kb[k] : nk;
+                if constexpr (HV) {
+                    const uint64_t nv = sv[lsw<uint64_t>(nx)];
+                    va[k] = takeA ? nv : va[k];
+                    vb[k] = takeA ? vb[k] : nv;
+                }
+            }
+        }
+    }
+}
+
 // Store IPT consecutive registers to LDS at item index base (a multiple of
 // the chunk size) as 16-byte stores.
 template <int IPT, bool HV, typename K>

@@ -76,8 +76,7 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
         const int o = tid * IPT;
         const int pb = o & ~(2 * R - 1);
         const int diag = o - pb;
-        const int a = lds_corank<K>(sk, pb, R, pb + R, R, diag);
-        lds_merge_serial<IPT, HV>(sk, sv, pb, R, pb + R, R, a, diag - a, IPT, TILE - 1, r, v);
+        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, pb, R, pb + R, R, diag, IPT, TILE - 1, r, v);
     }
     __syncthreads();
     lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
@@ -238,8 +237,7 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
     uint64_t v[IPT];
     if (d < cnt) {
         const int c = cnt - d < IPT ? cnt - d : IPT;
-        const int a = lds_corank<K>(sk, 0, la, la, lb, d);
-        lds_merge_serial<IPT, HV>(sk, sv, 0, la, la, lb, a, d - a, c, cnt - 1, r, v);
+        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, 0, la, la, lb, d, c, cnt - 1, r, v);
     }
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
@@ -651,8 +649,7 @@ __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restric
     uint64_t v[IPT];
     if (d < cnt) {
         const int c = cnt - d < IPT ? cnt - d : IPT;
-        const int a = lds_corank<K>(k2, 0, nx, nx, cnt - nx, d);
-        lds_merge_serial<IPT, HV>(k2, v2, 0, nx, nx, cnt - nx, a, d - a, c, cnt - 1, r, v);
+        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(k2, v2, 0, nx, nx, cnt - nx, d, c, cnt - 1, r, v);
     }
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(k1, v1, d, r, v);

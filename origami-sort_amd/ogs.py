@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libogs.so")
+LIB_PATH = os.environ.get("OGS_LIB") or os.path.join(HERE, "libogs.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ogs.h")
 
 U32, I32, U64, I64, F32, KV_U64U64, KV_I64I64 = range(7)
