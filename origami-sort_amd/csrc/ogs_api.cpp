@@ -232,20 +232,23 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
         (void)hipEventRecord(ev[1], s);
         prof.pending.push_back({PK_BLOCK, ev[0], ev[1], n, true});
     }
-    // co-rank tables: 2-way tile co-ranks, or for 4-way passes the sample
-    // table Q (every S outputs) followed by the per-tile split table
+    // co-rank tables: 2-way tile co-ranks, and for 4-way passes the tile
+    // split table (4 entries per slot; the first 4-way pass has the most)
     const uint64_t MT = ms.tile();
-    const uint32_t S = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(2048, MT), 2 * T);
-    uint64_t *part = nullptr, *qtab = nullptr;
+    uint64_t *part = nullptr;
     const bool table = tune[ki.kc].v[TP_PLAIN_CORANK] == 0;
     if (!one && (table || quad)) {
         Workspace *w = workspace(s);
         std::lock_guard<std::mutex> g(w->mu);
-        const uint64_t ntile = (n + MT - 1) / MT, nq = (n + S - 1) / S;
-        int rc = w->reserve_part((std::max(ntile, nq) + 3 * ntile + 8) * sizeof(uint64_t));
+        uint64_t need = (n + MT - 1) / MT + 4;
+        uint64_t R = T;
+        for (int wy : ways) {
+            if (wy == 4) need = std::max<uint64_t>(need, 4 * quad_table_slots(n, R, MT) + 8);
+            R *= (uint64_t)wy;
+        }
+        int rc = w->reserve_part(need * sizeof(uint64_t));
         if (rc) return rc;
         part = (uint64_t *)w->part;
-        qtab = part + std::max(ntile, nq) + 4;
     }
     if (!one) {
         uint64_t R = T;
@@ -257,8 +260,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
                 (void)hipEventRecord(ev[0], s);
             }
             if (wy == 4)
-                e = ms.run_quad(src, dst, n, R, last ? ki.codec : 0, part, qtab, S, tune[ki.kc].v[TP_PARTITION_LANES],
-                                s, prof.on ? ev[1] : nullptr);
+                e = ms.run_quad(src, dst, n, R, last ? ki.codec : 0, part, s, prof.on ? ev[1] : nullptr);
             else
                 e = ms.run(src, dst, n, R, last ? ki.codec : 0, table ? part : nullptr,
                            tune[ki.kc].v[TP_PARTITION_LANES], s, prof.on ? ev[1] : nullptr);

@@ -143,7 +143,7 @@ struct Server {
 };
 
 void Server::schedule_loop() {
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         cv_sched.wait(lk, [&] {
@@ -273,7 +273,7 @@ void Server::finish(Batch *b) {
 }
 
 void Server::complete_loop() {
-    hipSetDevice(ctx->device);
+    (void)hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         cv_inflight.wait(lk, [&] { return !inflight.empty() || sched_done; });

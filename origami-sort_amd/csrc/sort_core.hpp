@@ -77,9 +77,21 @@ struct GMem {
 // stores then hits distinct banks from its neighbours (plain layout: lanes 0
 // and 2 collide), and lane-consecutive accesses stay a permutation within a
 // row.  Items of width T; the swizzle never moves an item out of its chunk.
+#ifndef OGS_LDS_SWIZZLE
+#define OGS_LDS_SWIZZLE 1
+#endif
+// 1: LDS merges use per-thread co-ranks plus a bitonic slice merge
+// (lds_merge_bitonic); 0: serial merge chains (lds_merge_chains).
+#ifndef OGS_MERGE_BITONIC
+#define OGS_MERGE_BITONIC 0
+#endif
 template <typename T> __device__ __forceinline__ int lsw(int i) {
+#if OGS_LDS_SWIZZLE
     if constexpr (sizeof(T) == 4) return i ^ (((i >> 5) & 7) << 2);
     else return i ^ (((i >> 4) & 7) << 1);
+#else
+    return i;
+#endif
 }
 // items per 16-byte chunk
 template <typename T> constexpr int chunk_items() { return 16 / (int)sizeof(T); }
@@ -226,12 +238,71 @@ __device__ __forceinline__ int lds_corank(const K *sk, int a_off, int la, int b_
     return lo;
 }
 
-// Serial branch-free merge of `cnt` (<= IPT) outputs into registers starting at
-// A index ai and B index bi.  Reads one LDS item per output.  `limit` bounds
-// the LDS index read ahead of an exhausted side.
+// The same co-rank by binary lifting: a fixed, wave-uniform number of steps
+// (one per bit of `top`, a power of two with 2*top > min(la, lb)) and no
+// data-dependent branch, so every lane issues the same instructions.  Step s
+// tries to advance the answer by s (clamped to the upper bound) and keeps the
+// move when the last A item it would take still precedes the B item at the
+// same diagonal.  Reads A[-1] at most (a guard item before the array) and
+// B[lb] at most (slack after the array).
+template <typename K>
+__device__ __forceinline__ int lds_corank_lift(const K *sk, int a_off, int la, int b_off, int lb, int diag, int top) {
+    const int lo = diag > lb ? diag - lb : 0;
+    const int hi = diag < la ? diag : la;
+    int a = lo;
+    for (int s = top; s > 0; s >>= 1) {
+        const int t = min(a + s, hi);
+        const K ka = sk[lsw<K>(a_off + t - 1)];
+        const K kb = sk[lsw<K>(b_off + diag - t)];
+        a = kb < ka ? a : t;
+    }
+    return a;
+}
+
+// Items of LDS guard space around a merge buffer: one 16-byte chunk before it
+// (the co-rank reads A[-1]) and IPT items after it (the bitonic slice merge
+// reads up to IPT-1 items past either run).
+template <typename K> constexpr int lds_front() { return 16 / (int)sizeof(K); }
+template <typename K, int IPT> constexpr int lds_extra() { return lds_front<K>() + IPT; }
+
+// Merges this thread's slice of merge(A, B) -- `na` items of A starting at
+// LDS index ia, and IPT - na items of B ending at LDS index jb - na (inclusive)
+// -- into IPT sorted registers.  The slice is laid out as A ascending followed
+// by B descending (w[i] = i < na ? A[ia + i] : B[jb - i]), which is bitonic, and
+// sorted by a log2(IPT)-stage bitonic merge.  All loads are independent of one
+// another (no per-output dependency chain), and with an unswizzled layout they
+// are base + constant offsets.  Ties between equal keys may leave in either
+// order; the set of items is exactly the slice.
+template <int IPT, bool HV, typename K>
+__device__ __forceinline__ void lds_merge_bitonic(const K *sk, const uint64_t *sv, int ia, int na, int jb,
+                                                  K (&r)[IPT], uint64_t (&v)[IPT]) {
+    static_assert((IPT & (IPT - 1)) == 0, "bitonic slice merge needs a power-of-two IPT");
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+        if constexpr (HV) {
+            const int idx = i < na ? ia + i : jb - i;
+            r[i] = sk[lsw<K>(idx)];
+            v[i] = sv[lsw<uint64_t>(idx)];
+        } else {
+            const K ka = sk[lsw<K>(ia + i)];
+            const K kb = sk[lsw<K>(jb - i)];
+            r[i] = i < na ? ka : kb;
+            v[i] = 0;
+        }
+    }
+#pragma unroll
+    for (int s = IPT / 2; s > 0; s >>= 1)
+#pragma unroll
+        for (int i = 0; i < IPT; i++)
+            if (!(i & s)) cex<HV>(r[i], r[i + s], v[i], v[i + s]);
+}
+
+// Serial branch-free merge of `cnt` outputs into registers r[i0 .. i0+cnt)
+// (i0 + cnt <= IPT) starting at A index ai and B index bi.  Reads one LDS item
+// per output.  `limit` bounds the LDS index read ahead of an exhausted side.
 template <int IPT, bool HV, typename K>
 __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv, int a_off, int la, int b_off,
-                                                 int lb, int ai, int bi, int cnt, int limit, K (&r)[IPT],
+                                                 int lb, int ai, int bi, int i0, int cnt, int limit, K (&r)[IPT],
                                                  uint64_t (&v)[IPT]) {
     int ia = a_off + ai, ib = b_off + bi;
     const int ea = a_off + la, eb = b_off + lb;
@@ -243,7 +314,7 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
     }
 #pragma unroll
     for (int i = 0; i < IPT; i++) {
-        if (i < cnt) {
+        if (i >= i0 && i < i0 + cnt) {
             bool takeA = ib >= eb || (ia < ea && !(kb < ka));
             r[i] = takeA ? ka : kb;
             if constexpr (HV) v[i] = takeA ? va : vb;
@@ -267,7 +338,7 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
 // co-rank search), and the chains advance in lockstep, so each step issues CH
 // independent LDS reads instead of one dependent read per output.
 #ifndef OGS_MERGE_CHAINS
-#define OGS_MERGE_CHAINS 2
+#define OGS_MERGE_CHAINS 1
 #endif
 template <int IPT, int CH, bool HV, typename K>
 __device__ __forceinline__ void lds_merge_chains(const K *sk, const uint64_t *sv, int a_off, int la, int b_off,

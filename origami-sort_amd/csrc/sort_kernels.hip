@@ -23,7 +23,7 @@ namespace ogs {
 // ----------------------------------------------------------------------------
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t base, int cnt, int cin, int cout,
-                                          K *sk, uint64_t *sv, int *smisc) {
+                                          K *sk, uint64_t *sv, int *smisc, int *sa) {
     constexpr int TILE = BLOCK * IPT;
     constexpr K KMAX = KeyMax<K>::v;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -76,7 +76,15 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
         const int o = tid * IPT;
         const int pb = o & ~(2 * R - 1);
         const int diag = o - pb;
+#if OGS_MERGE_BITONIC
+        const int a = lds_corank_lift<K>(sk, pb, R, pb + R, R, diag, R);
+        sa[tid] = a;
+        __syncthreads();
+        const int ae = diag + IPT == 2 * R ? R : sa[tid + 1];
+        lds_merge_bitonic<IPT, HV>(sk, sv, pb + a, ae - a, pb + R + diag - a + IPT - 1, r, v);
+#else
         lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, pb, R, pb + R, R, diag, IPT, TILE - 1, r, v);
+#endif
     }
     __syncthreads();
     lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
@@ -87,12 +95,14 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out, uint64_t n, int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ int smisc[2];
+    __shared__ int sa[BLOCK + 1];
+    K *sk = skb + lds_front<K>();
     const uint64_t base = (uint64_t)blockIdx.x * TILE;
     const int cnt = (int)(n - base < (uint64_t)TILE ? n - base : (uint64_t)TILE);
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc);
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc, sa);
 }
 
 // Batched tile sort: descriptor i sorts one tile -- a whole small segment, or
@@ -110,12 +120,15 @@ static_assert(sizeof(TileDesc) == 24, "descriptor layout is shared with the host
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const TileDesc *__restrict__ tiles, int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ int smisc[2];
+    __shared__ int sa[BLOCK + 1];
+    K *sk = skb + lds_front<K>();
     const TileDesc t = tiles[blockIdx.x];
     const int cnt = (int)(t.cnt & 0x3FFFFFFFu);
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, ((t.cnt >> 30) & 1u) ? cout : 0, sk, sv, smisc);
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, ((t.cnt >> 30) & 1u) ? cout : 0, sk, sv, smisc,
+                                       sa);
 }
 
 // ----------------------------------------------------------------------------
@@ -193,7 +206,7 @@ __global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint6
 template <typename K, bool HV, int BLOCK, int IPT>
 __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__restrict__ out, uint64_t seg,
                                            uint64_t n, uint64_t R, uint64_t o0, int cout, K *sk, uint64_t *sv,
-                                           uint64_t *sco, const uint64_t *part) {
+                                           uint64_t *sco, int *sa, const uint64_t *part) {
     static_assert(BLOCK >= 128, "the co-rank searches need two waves");
     constexpr int TILE = BLOCK * IPT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -235,10 +248,26 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
     const int d = tid * IPT;
     K r[IPT];
     uint64_t v[IPT];
+#if OGS_MERGE_BITONIC
+    // One co-rank per thread, exchanged through LDS so each thread knows where
+    // its slice ends; the one thread whose slice is partial merges serially.
+    int top = la < lb ? la : lb;
+    top = top ? 1 << (31 - __clz(top)) : 0;
+    const int a = d < cnt ? lds_corank_lift<K>(sk, 0, la, la, lb, d, top) : 0;
+    sa[tid] = a;
+    __syncthreads();
+    if (d + IPT <= cnt) {
+        const int ae = d + IPT == cnt ? la : sa[tid + 1];
+        lds_merge_bitonic<IPT, HV>(sk, sv, a, ae - a, la + d - a + IPT - 1, r, v);
+    } else if (d < cnt) {
+        lds_merge_serial<IPT, HV>(sk, sv, 0, la, la, lb, a, d - a, 0, cnt - d, cnt - 1, r, v);
+    }
+#else
     if (d < cnt) {
         const int c = cnt - d < IPT ? cnt - d : IPT;
         lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, 0, la, la, lb, d, c, cnt - 1, r, v);
     }
+#endif
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
     __syncthreads();
@@ -249,10 +278,12 @@ template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
                                                       uint64_t R, int cout, const uint64_t *__restrict__ part) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ uint64_t sco[2];
-    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco,
+    __shared__ int sa[BLOCK + 1];
+    K *sk = skb + lds_front<K>();
+    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco, sa,
                                   part ? part + blockIdx.x : nullptr);
 }
 
@@ -292,13 +323,15 @@ template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__restrict__ tiles, uint64_t R, int cout,
                                                           const uint64_t *__restrict__ part) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ __attribute__((aligned(16))) K sk[TILE];
+    __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
     __shared__ uint64_t sco[2];
+    __shared__ int sa[BLOCK + 1];
+    K *sk = skb + lds_front<K>();
     const MergeDesc d = tiles[blockIdx.x];
     const int co = (d.o0 & kLastBit) ? cout : 0;
     const uint64_t o0 = d.o0 & ~kLastBit;
-    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco,
+    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco, sa,
                                   part ? part + blockIdx.x : nullptr);
 }
 
@@ -387,17 +420,25 @@ static hipError_t launch_seg_merge(const void *tiles, uint64_t ntiles, uint64_t 
 
 
 // ----------------------------------------------------------------------------
-// 4-way merge pass: groups of four runs A, B, C, D (length R each) become one
-// run of 4R in a single HBM pass -- two levels of the reference's merge tree
-// (MergeTreeEven, merge_tree.h:2576-3010) per read and write of the data.
-// The output tile [r, r+T) of a group needs the exact split of all four runs.
-// With X = merge(A, B) and Y = merge(C, D), the split is the merge-path
-// co-rank x of (X, Y) at r followed by the co-ranks of (A, B) at x and of
-// (C, D) at r - x.  X and Y are never materialised: a 2-way co-rank table Q
-// sampled every S outputs (the 2-way partition kernel run with tile S) gives
-// X[jS] and Y[qS-1] directly, so a coarse search over multiples of S brackets
-// x to one S-window, and a fine search inside it evaluates X[x] and Y[y]
-// through co-ranks confined to one sample interval (<= log2(S)+1 steps).
+// 4-way merge pass: groups of four runs A, B, C, D (length R each, ragged in
+// the last group) become one run of 4R in a single HBM pass -- two levels of
+// the reference's merge tree (MergeTreeEven, merge_tree.h:2576-3010) per read
+// and write of the data.  With X = merge(A, B) and Y = merge(C, D) (ties go
+// A, B, C, D), the output is merge(X, Y).
+//
+// Tiles are cut at points of the (X, Y) merge path: one at every h-th item of
+// X, one at every h-th item of Y (h = TILE/2), and the group start.  Between
+// consecutive points X and Y each advance by at most h, so a tile holds at
+// most TILE items, and every point splits all four runs exactly:
+//  - the point at X index x: (a, b) = co-rank of (A, B) at x, v = X[x], and Y
+//    contributes its items below v: c = #C < v, d = #D < v;
+//  - the point at Y index y: (c, d) = co-rank of (C, D) at y, w = Y[y], and X
+//    contributes its items up to w: a = #A <= w, b = #B <= w.
+// Along the path the X point j is preceded by exactly ceil((c+d)/h) Y points
+// (a Y point at the same y follows it), and the Y point i by ceil((a+b)/h) X
+// points, so each point's slot in its group's tile list is known in closed
+// form and no sort of the points is needed.  Every X and Y index is found with
+// one 2-way co-rank and two independent binary searches.
 // ----------------------------------------------------------------------------
 struct QuadGeom {
     uint64_t g4;              // group base
@@ -415,264 +456,196 @@ __device__ __forceinline__ QuadGeom quad_geom(uint64_t o, uint64_t n, uint64_t R
     return q;
 }
 
-// co-rank(x) of merge(A, B) (ties to A), confined by the sample table:
-// Q[qbase + j] = co-rank at j*S for j*S < la + lb.
+// Tile slots per group: the group start, 2R/h X points and 2R/h Y points
+// (h divides R).  The last group uses the first 1 + ceil(lx/h) + ceil(ly/h).
+__host__ __device__ inline uint64_t quad_slots(uint64_t R, uint64_t h) { return 1 + 4 * R / h; }
+
+// Per-thread merge-path co-rank in global memory (ties to A).
 template <typename K, bool HV>
-__device__ __forceinline__ uint64_t pair_corank_win(const void *in, uint64_t abase, uint64_t la, uint64_t bbase,
-                                                    uint64_t lb, const uint64_t *Q, uint64_t qbase, uint32_t S,
-                                                    uint64_t x) {
-    const uint64_t js = x / S;
-    const uint64_t alo = Q[qbase + js];
-    uint64_t u = (js + 1) * S, au;
-    if (u >= la + lb) {
-        u = la + lb;
-        au = la;
-    } else {
-        au = Q[qbase + js + 1];
-    }
-    // co-rank grows by 0 or 1 per output
-    uint64_t lo = alo, hi = au;
-    if (au > u - x && au - (u - x) > lo) lo = au - (u - x);
-    if (alo + (x - js * S) < hi) hi = alo + (x - js * S);
-    if (x > lb && x - lb > lo) lo = x - lb;
-    if (x < hi) hi = x;
-    if (la < hi) hi = la;
+__device__ __forceinline__ uint64_t thread_corank(const void *in, uint64_t abase, uint64_t la, uint64_t bbase,
+                                                  uint64_t lb, uint64_t diag) {
+    uint64_t lo = diag > lb ? diag - lb : 0;
+    uint64_t hi = diag < la ? diag : la;
     while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
         const K a = GMem<K, HV>::load_key(in, abase + mid);
-        const K b = GMem<K, HV>::load_key(in, bbase + x - 1 - mid);
+        const K b = GMem<K, HV>::load_key(in, bbase + diag - 1 - mid);
         if (!(b < a)) lo = mid + 1;
         else hi = mid;
     }
     return lo;
 }
 
+// The item after taking a items of run P and b items of run Q in merge(P, Q)
+// (ties to P); at least one of them has an item left.
 template <typename K, bool HV>
-__device__ __forceinline__ K key_after_split(const void *in, uint64_t abase, uint64_t la, uint64_t bbase, uint64_t lb,
-                                             uint64_t a, uint64_t b) {
-    // the next output after taking a items of A and b items of B
-    if (a >= la) return GMem<K, HV>::load_key(in, bbase + b);
-    if (b >= lb) return GMem<K, HV>::load_key(in, abase + a);
-    const K ka = GMem<K, HV>::load_key(in, abase + a);
-    const K kb = GMem<K, HV>::load_key(in, bbase + b);
-    return (kb < ka) ? kb : ka;
+__device__ __forceinline__ K next_merged(const void *in, uint64_t pbase, uint64_t lp, uint64_t qbase, uint64_t lq,
+                                         uint64_t a, uint64_t b) {
+    if (b >= lq) return GMem<K, HV>::load_key(in, pbase + a);
+    if (a >= lp) return GMem<K, HV>::load_key(in, qbase + b);
+    const K kp = GMem<K, HV>::load_key(in, pbase + a);
+    const K kq = GMem<K, HV>::load_key(in, qbase + b);
+    return kq < kp ? kq : kp;
 }
 
+// Items of runs P and Q below v (upper = false) or up to v (upper = true):
+// two independent binary searches advanced in lockstep.
 template <typename K, bool HV>
-__device__ __forceinline__ K key_before_split(const void *in, uint64_t abase, uint64_t bbase, uint64_t a, uint64_t b) {
-    // the last output after taking a items of A and b items of B (a + b > 0)
-    if (a == 0) return GMem<K, HV>::load_key(in, bbase + b - 1);
-    if (b == 0) return GMem<K, HV>::load_key(in, abase + a - 1);
-    const K ka = GMem<K, HV>::load_key(in, abase + a - 1);
-    const K kb = GMem<K, HV>::load_key(in, bbase + b - 1);
-    return (kb < ka) ? ka : kb;
-}
-
-// Smallest index in [lo, hi) whose predicate holds, else hi, searched by a
-// group of LANES lanes (monotone predicate; all lanes of the group call it).
-template <int LANES, typename Pred>
-__device__ __forceinline__ uint64_t group_first_true(uint64_t lo, uint64_t hi, int l, int gshift, Pred pred) {
-    while (hi - lo > (uint64_t)LANES) {
-        const uint64_t span = hi - lo;
-        const uint64_t m = lo + (span * (uint64_t)(l + 1)) / (LANES + 1);
-        const uint64_t bits = (__ballot(pred(m)) >> gshift) & ((1ull << LANES) - 1);
-        if (!bits) {
-            lo = lo + (span * LANES) / (LANES + 1) + 1;
-        } else {
-            const int f = __ffsll((unsigned long long)bits) - 1;
-            const uint64_t mf = lo + (span * (uint64_t)(f + 1)) / (LANES + 1);
-            lo = f ? lo + (span * (uint64_t)f) / (LANES + 1) + 1 : lo;
-            hi = mf;
+__device__ __forceinline__ void rank_pair(const void *in, uint64_t pbase, uint64_t lp, uint64_t qbase, uint64_t lq,
+                                          K v, bool upper, uint64_t &rp, uint64_t &rq) {
+    uint64_t lo0 = 0, hi0 = lp, lo1 = 0, hi1 = lq;
+    while (lo0 < hi0 || lo1 < hi1) {
+        const uint64_t m0 = (lo0 + hi0) >> 1, m1 = (lo1 + hi1) >> 1;
+        const bool on0 = lo0 < hi0, on1 = lo1 < hi1;
+        K k0 = v, k1 = v;
+        if (on0) k0 = GMem<K, HV>::load_key(in, pbase + m0);
+        if (on1) k1 = GMem<K, HV>::load_key(in, qbase + m1);
+        if (on0) {
+            if (upper ? !(v < k0) : k0 < v) lo0 = m0 + 1;
+            else hi0 = m0;
+        }
+        if (on1) {
+            if (upper ? !(v < k1) : k1 < v) lo1 = m1 + 1;
+            else hi1 = m1;
         }
     }
-    bool p = false;
-    if ((uint64_t)l < hi - lo) p = pred(lo + l);
-    const uint64_t bits = (__ballot(p) >> gshift) & ((1ull << LANES) - 1);
-    return bits ? lo + (uint64_t)(__ffsll((unsigned long long)bits) - 1) : hi;
+    rp = lo0;
+    rq = lo1;
 }
 
-// One group of LANES lanes per 4-way output tile: tab[3t..3t+2] = (a, x, c),
-// the A items, X = A+B items and C items before the tile's first output.
-template <typename K, bool HV, int LANES>
-__global__ void k_quad_partition(const void *__restrict__ in, uint64_t n, uint64_t R, uint32_t T, uint32_t S,
-                                 uint64_t ntiles, const uint64_t *__restrict__ Q, uint64_t *__restrict__ tab) {
-    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int l = (int)(gt % LANES);
-    const int gshift = (int)(threadIdx.x & 63) & ~(LANES - 1);
-    uint64_t t = gt / LANES;
-    const bool live = t < ntiles;
-    if (!live) t = ntiles - 1;
-    const uint64_t o = t * T;
-    const QuadGeom q = quad_geom(o, n, R);
-    const uint64_t r = o - q.g4;
+// One thread per tile slot: writes the slot's split (a, b, c, d) of its group
+// to tab[4*slot ..].
+template <typename K, bool HV>
+__global__ void k_quad_bounds(const void *__restrict__ in, uint64_t n, uint64_t R, uint64_t h, uint64_t nslots,
+                              uint64_t *__restrict__ tab) {
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslots) return;
+    const uint64_t per = quad_slots(R, h);
+    const uint64_t g = s / per, k = s % per;
+    const QuadGeom q = quad_geom(g * 4 * R, n, R);
     const uint64_t lx = q.la + q.lb, ly = q.lc + q.ld;
-    const uint64_t A = q.g4, B = q.g4 + q.la, C = q.g4 + lx, D = q.g4 + lx + q.lc;
-    const uint64_t qab = q.g4 / S, qcd = (q.g4 + 2 * R) / S;
-    const uint64_t xlo = r > ly ? r - ly : 0, xhi = r < lx ? r : lx;
-    uint64_t x = xhi;
-    if (xlo < xhi) {
-        // coarse: candidates x = jS in [xlo, xhi); P(jS): X[jS] > Y[r - jS - 1]
-        const uint64_t jlo = (xlo + S - 1) / S, jhi1 = (xhi - 1) / S + 1;  // j in [jlo, jhi1)
-        uint64_t wlo = xlo, whi = xhi;
-        if (jlo < jhi1) {
-            auto coarse = [&](uint64_t j) {
-                const uint64_t xs = j * S;
-                const uint64_t a = Q[qab + j];
-                const K kx = key_after_split<K, HV>(in, A, q.la, B, q.lb, a, xs - a);
-                const uint64_t ys = r - xs;  // multiple of S, 1 <= ys <= ly
-                const uint64_t c = ys >= ly ? q.lc : Q[qcd + ys / S];
-                const K ky = key_before_split<K, HV>(in, C, D, c, ys - c);
-                return ky < kx;
-            };
-            const uint64_t jt = group_first_true<LANES>(jlo, jhi1, l, gshift, coarse);
-            if (jt < jhi1) {
-                whi = jt * S;
-                wlo = jt > jlo ? (jt - 1) * S + 1 : xlo;
-            } else {
-                wlo = (jhi1 - 1) * S + 1;
-            }
-        }
-        // fine: smallest x in [wlo, whi) with X[x] > Y[r - 1 - x], else whi
-        auto fine = [&](uint64_t xx) {
-            const uint64_t a = pair_corank_win<K, HV>(in, A, q.la, B, q.lb, Q, qab, S, xx);
-            const K kx = key_after_split<K, HV>(in, A, q.la, B, q.lb, a, xx - a);
-            const uint64_t yy = r - 1 - xx;  // Y[yy] = the key after yy outputs of Y
-            const uint64_t c = pair_corank_win<K, HV>(in, C, q.lc, D, q.ld, Q, qcd, S, yy);
-            const K ky = key_after_split<K, HV>(in, C, q.lc, D, q.ld, c, yy - c);
-            return ky < kx;
-        };
-        x = wlo < whi ? group_first_true<LANES>(wlo, whi, l, gshift, fine) : wlo;
+    const uint64_t A = q.g4, B = A + q.la, C = A + lx, D = C + q.lc;
+    const uint64_t half = 2 * R / h;
+    uint64_t a = 0, b = 0, c = 0, d = 0, slot = 0;
+    if (k > 0 && k - 1 < half) {
+        const uint64_t j = k - 1, x = j * h;
+        if (x >= lx) return;
+        a = thread_corank<K, HV>(in, A, q.la, B, q.lb, x);
+        b = x - a;
+        const K v = next_merged<K, HV>(in, A, q.la, B, q.lb, a, b);
+        rank_pair<K, HV>(in, C, q.lc, D, q.ld, v, false, c, d);
+        slot = 1 + j + (c + d + h - 1) / h;
+    } else if (k > 0) {
+        const uint64_t i = k - 1 - half, y = i * h;
+        if (y >= ly) return;
+        c = thread_corank<K, HV>(in, C, q.lc, D, q.ld, y);
+        d = y - c;
+        const K w = next_merged<K, HV>(in, C, q.lc, D, q.ld, c, d);
+        rank_pair<K, HV>(in, A, q.la, B, q.lb, w, true, a, b);
+        slot = 1 + i + (a + b + h - 1) / h;
     }
-    const uint64_t a = lx ? pair_corank_win<K, HV>(in, A, q.la, B, q.lb, Q, qab, S, x < lx ? x : lx - 1) : 0;
-    const uint64_t a_fix = (x == lx) ? q.la : a;
-    const uint64_t y = r - x;
-    const uint64_t c = ly ? pair_corank_win<K, HV>(in, C, q.lc, D, q.ld, Q, qcd, S, y < ly ? y : ly - 1) : 0;
-    const uint64_t c_fix = (y == ly) ? q.lc : c;
-    if (live && l == 0) {
-        tab[3 * t + 0] = a_fix;
-        tab[3 * t + 1] = x;
-        tab[3 * t + 2] = c_fix;
-    }
+    ulonglong2 *t = (ulonglong2 *)(tab + 4 * (g * per + slot));
+    t[0] = make_ulonglong2(a, b);
+    t[1] = make_ulonglong2(c, d);
 }
 
-// Serial merge of `count` outputs of merge(A, B) starting at diagonal diag,
-// written to LDS (dk/dv) at out_pos.. (runtime count).
-template <bool HV, typename K>
-__device__ __forceinline__ void lds_merge_to_lds(const K *sk, const uint64_t *sv, int a_off, int la, int b_off, int lb,
-                                                 int diag, int count, int limit, K *dk, uint64_t *dv, int out_pos) {
-    const int a = lds_corank<K>(sk, a_off, la, b_off, lb, diag);
-    int ia = a_off + a, ib = b_off + (diag - a);
-    const int ea = a_off + la, eb = b_off + lb;
-    K ka = sk[lsw<K>(min(ia, limit))], kb = sk[lsw<K>(min(ib, limit))];
-    uint64_t va = 0, vb = 0;
-    if constexpr (HV) {
-        va = sv[lsw<uint64_t>(min(ia, limit))];
-        vb = sv[lsw<uint64_t>(min(ib, limit))];
-    }
-    for (int i = 0; i < count; i++) {
-        const bool takeA = ib >= eb || (ia < ea && !(kb < ka));
-        dk[lsw<K>(out_pos + i)] = takeA ? ka : kb;
-        if constexpr (HV) dv[lsw<uint64_t>(out_pos + i)] = takeA ? va : vb;
-        ia += takeA;
-        ib += !takeA;
-        const int nx = min(takeA ? ia : ib, limit);
-        const K nk = sk[lsw<K>(nx)];
-        ka = takeA ? nk : ka;
-        kb = takeA ? kb : nk;
-        if constexpr (HV) {
-            const uint64_t nv = sv[lsw<uint64_t>(nx)];
-            va = takeA ? nv : va;
-            vb = takeA ? vb : nv;
-        }
-    }
-}
-
+// One workgroup per tile slot: loads the tile's four slices into LDS, merges
+// A+B and C+D into one LDS array (level 1: each thread's IPT outputs may span
+// the X/Y boundary), merges that X and Y (level 2), and writes the tile with
+// lane-consecutive stores.  Slots past the end of a ragged group, and empty
+// tiles, exit at once.
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restrict__ in, void *__restrict__ out,
                                                            uint64_t n, uint64_t R, int cout,
                                                            const uint64_t *__restrict__ tab) {
     constexpr int TILE = BLOCK * IPT;
-    __shared__ __attribute__((aligned(16))) K sk[2 * TILE];
-    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? 2 * TILE : 2];
-    K *k1 = sk, *k2 = sk + TILE;
-    uint64_t *v1 = sv, *v2 = HV ? sv + TILE : sv;
+    constexpr uint64_t h = TILE / 2;
+    __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    K *sk = skb + lds_front<K>();
     const int tid = threadIdx.x;
-    const uint64_t t = blockIdx.x;
-    const uint64_t o0 = t * TILE;
-    const QuadGeom q = quad_geom(o0, n, R);
-    const uint64_t r0 = o0 - q.g4;
-    const int cnt = (int)(n - o0 < (uint64_t)TILE ? n - o0 : (uint64_t)TILE);
-    const uint64_t r1 = r0 + cnt;
+    const uint64_t per = quad_slots(R, h);
+    const uint64_t s = blockIdx.x, g = s / per, k = s % per;
+    const QuadGeom q = quad_geom(g * 4 * R, n, R);
     const uint64_t lx = q.la + q.lb, ly = q.lc + q.ld;
-    const uint64_t a0 = tab[3 * t], x0 = tab[3 * t + 1], c0 = tab[3 * t + 2];
-    uint64_t a1, x1, c1;
-    if (r1 == lx + ly) {
-        a1 = q.la;
-        x1 = lx;
-        c1 = q.lc;
-    } else {
-        a1 = tab[3 * t + 3];
-        x1 = tab[3 * t + 4];
-        c1 = tab[3 * t + 5];
+    const uint64_t used = 1 + (lx + h - 1) / h + (ly + h - 1) / h;
+    if (k >= used) return;
+    const uint64_t *t0 = tab + 4 * s;
+    const uint64_t a0 = t0[0], b0 = t0[1], c0 = t0[2], d0 = t0[3];
+    uint64_t a1 = q.la, b1 = q.lb, c1 = q.lc, d1 = q.ld;
+    if (k + 1 < used) {
+        a1 = t0[4];
+        b1 = t0[5];
+        c1 = t0[6];
+        d1 = t0[7];
     }
-    const uint64_t b0 = x0 - a0, b1 = x1 - a1, y0 = r0 - x0, y1 = r1 - x1, d0 = y0 - c0, d1 = y1 - c1;
+    // a split table that is not monotone or overfills the tile is never read
+    // past: such a tile is skipped (its output stays unwritten)
+    if (a0 > a1 || a1 > q.la || b0 > b1 || b1 > q.lb || c0 > c1 || c1 > q.lc || d0 > d1 || d1 > q.ld) return;
+    if ((a1 - a0) + (b1 - b0) + (c1 - c0) + (d1 - d0) > (uint64_t)TILE) return;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0), nc = (int)(c1 - c0), nd = (int)(d1 - d0);
+    const int cnt = na + nb + nc + nd;
+    if (cnt == 0) return;
     const uint64_t sA = q.g4 + a0, sB = q.g4 + q.la + b0, sC = q.g4 + lx + c0, sD = q.g4 + lx + q.lc + d0;
     const int e1 = na, e2 = na + nb, e3 = na + nb + nc;
 #pragma unroll 4
     for (int e = tid; e < cnt; e += BLOCK) {
         const uint64_t src = e < e1 ? sA + e : e < e2 ? sB + (e - e1) : e < e3 ? sC + (e - e2) : sD + (e - e3);
-        K k;
+        K key;
         uint64_t vv = 0;
-        GMem<K, HV>::load(in, src, k, vv);
-        k1[lsw<K>(e)] = k;
-        if constexpr (HV) v1[lsw<uint64_t>(e)] = vv;
+        GMem<K, HV>::load(in, src, key, vv);
+        sk[lsw<K>(e)] = key;
+        if constexpr (HV) sv[lsw<uint64_t>(e)] = vv;
     }
     __syncthreads();
-    // level 1: X part = merge(A, B) -> k2[0, nx), Y part = merge(C, D) -> k2[nx, cnt)
-    const int nx = na + nb;
-    const int d = tid * IPT;
-    if (d < cnt) {
-        const int c = cnt - d < IPT ? cnt - d : IPT;
-        if (d < nx) {
-            const int k = nx - d < c ? nx - d : c;
-            lds_merge_to_lds<HV>(k1, v1, 0, na, na, nb, d, k, cnt - 1, k2, v2, d);
-        }
-        if (d + c > nx) {
-            const int s0 = d > nx ? d : nx;
-            lds_merge_to_lds<HV>(k1, v1, e2, nc, e3, nd, s0 - nx, d + c - s0, cnt - 1, k2, v2, s0);
-        }
-    }
-    __syncthreads();
-    // level 2: merge(X, Y) -> registers -> LDS -> coalesced stores
+    const int nx = e2, d = tid * IPT;
     K r[IPT];
     uint64_t v[IPT];
     if (d < cnt) {
-        const int c = cnt - d < IPT ? cnt - d : IPT;
-        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(k2, v2, 0, nx, nx, cnt - nx, d, c, cnt - 1, r, v);
+        if (d < nx) {
+            const int cx = nx - d < IPT ? nx - d : IPT;
+            const int a = lds_corank<K>(sk, 0, na, na, nb, d);
+            lds_merge_serial<IPT, HV>(sk, sv, 0, na, na, nb, a, d - a, 0, cx, cnt - 1, r, v);
+        }
+        if (d + IPT > nx && nx < cnt) {
+            const int st = d > nx ? d : nx;
+            const int yd = st - nx;
+            const int cy = (d + IPT < cnt ? d + IPT : cnt) - st;
+            const int c = lds_corank<K>(sk, e2, nc, e3, nd, yd);
+            lds_merge_serial<IPT, HV>(sk, sv, e2, nc, e3, nd, c, yd - c, st - d, cy, cnt - 1, r, v);
+        }
     }
     __syncthreads();
-    if (d < cnt) lds_put<IPT, HV>(k1, v1, d, r, v);
+    if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
     __syncthreads();
-    lds_store_tile<BLOCK, TILE, HV>(k1, v1, out, o0, cnt, cout);
+    if (d < cnt) {
+        const int c = cnt - d < IPT ? cnt - d : IPT;
+        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, 0, nx, nx, cnt - nx, d, c, cnt - 1, r, v);
+    }
+    __syncthreads();
+    if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
+    __syncthreads();
+    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, q.g4 + a0 + b0 + c0 + d0, cnt, cout);
 }
 
-// Launches the three kernels of one 4-way pass.  Q: ceil(n/S) entries,
-// tab: 3*ceil(n/TILE) entries.
+// Slots of one 4-way pass over n items with runs of R and this tile size.
+__host__ __device__ inline uint64_t quad_pass_slots(uint64_t n, uint64_t R, uint64_t tile) {
+    return (n + 4 * R - 1) / (4 * R) * quad_slots(R, tile / 2);
+}
+
+// Launches the two kernels of one 4-way pass; tab holds 4 * quad_pass_slots
+// entries.  `mid` is recorded between them.
 template <typename K, bool HV, int BLOCK, int IPT>
-static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *Q, uint64_t *tab,
-                              uint32_t S, int lanes, hipStream_t s, hipEvent_t mid) {
+static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
+                              hipStream_t s, hipEvent_t mid) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
-    constexpr int QL = 16;
-    const uint64_t ntiles = (n + TILE - 1) / TILE;
-    if (ntiles == 0) return hipSuccess;
-    const uint64_t nq = (n + S - 1) / S;
-    launch_partition<K, HV>(lanes, in, n, R, S, nq, Q, s);
-    const uint64_t threads = ntiles * QL;
-    hipLaunchKernelGGL((k_quad_partition<K, HV, QL>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, in, n,
-                       R, (uint32_t)TILE, S, ntiles, (const uint64_t *)Q, tab);
+    if (n == 0) return hipSuccess;
+    if (R % (TILE / 2)) return hipErrorInvalidValue;
+    const uint64_t nslots = quad_pass_slots(n, R, TILE);
+    hipLaunchKernelGGL((k_quad_bounds<K, HV>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, in, n, R,
+                       TILE / 2, nslots, tab);
     if (mid) (void)hipEventRecord(mid, s);
-    hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n,
+    hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)nslots), dim3(BLOCK), 0, s, in, out, n,
                        R, cout, (const uint64_t *)tab);
     return hipGetLastError();
 }
@@ -681,7 +654,8 @@ static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R,
 // (BLOCK, IPT, WLEV), merge shapes (BLOCK, IPT).  The first entry of each list
 // is the default; the tuning table may pick any other.
 #define OGS_BLOCK_SHAPES_32(X) X(256, 16, 1) X(256, 16, 0) X(256, 16, 2) X(256, 16, 3) X(256, 32, 1) \
-    X(512, 16, 1) X(512, 32, 1) X(1024, 16, 1) X(1024, 32, 1) X(128, 16, 1) X(256, 8, 1) X(64, 16, 1) X(64, 8, 1)
+    X(512, 16, 1) X(512, 32, 1) X(1024, 16, 1) X(1024, 32, 1) X(128, 16, 1) X(256, 8, 1) X(64, 16, 1) X(64, 8, 1) \
+    X(512, 16, 0) X(1024, 16, 0) X(512, 32, 0)
 #define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8) X(1024, 16) \
     X(512, 32) X(128, 4)
 #define OGS_BLOCK_SHAPES_64(X) X(256, 16, 1) X(256, 8, 1) X(128, 16, 1) X(512, 8, 1) X(512, 16, 1) \

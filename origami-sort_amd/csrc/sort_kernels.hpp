@@ -48,11 +48,15 @@ struct BlockShapeFns {
     SegBlockLaunch run_seg;
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
-// One 4-way pass (runs of R -> 4R): the 2-way co-rank sample table Q
-// (ceil(n/S) entries, S | 2R and S | tile), the per-tile split table tab
-// (3*ceil(n/tile) entries), then the merge; `mid` is recorded before the merge.
-typedef hipError_t (*QuadLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *Q,
-                                 uint64_t *tab, uint32_t S, int lanes, hipStream_t s, hipEvent_t mid);
+// One 4-way pass (runs of R -> 4R, tile/2 dividing R): the tile split table
+// tab (4 * quad_table_slots entries) is filled, then the merge runs; `mid` is
+// recorded between the two.
+typedef hipError_t (*QuadLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
+                                 hipStream_t s, hipEvent_t mid);
+// Tile slots of one 4-way pass (groups of 4R, 1 + 8R/tile slots each).
+inline uint64_t quad_table_slots(uint64_t n, uint64_t R, uint64_t tile) {
+    return (n + 4 * R - 1) / (4 * R) * (1 + 8 * R / tile);
+}
 
 struct MergeShapeFns {
     int block, ipt;

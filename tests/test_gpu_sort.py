@@ -269,8 +269,9 @@ def test_segmented_tiny_class(ctx):
 @pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32, ogs.I64])
 @pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.DISTINCT16, ogs.SORTED, ogs.REVERSE, ogs.ZIPF])
 def test_four_way_passes(ctx, kind, dist):
-    # plain_fanin 4: two merge levels per HBM pass (sample table + exact split
-    # search + two LDS merge levels); odd level counts end with a 2-way pass
+    # plain_fanin 4: two merge levels per HBM pass (tiles cut at merge-path
+    # points every TILE/2 items of each pair's merge, two LDS merge levels);
+    # odd level counts end with a 2-way pass
     bshapes = ogs.shapes(kind, 0)
     old = {p: ctx.tune_get(kind, p) for p in ("plain_fanin", "plain_block", "plain_merge")}
     try:
