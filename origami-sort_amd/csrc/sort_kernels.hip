@@ -30,20 +30,20 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     K r[IPT];
     uint64_t v[IPT];
     int nmax = 0;
+    // all IPT loads are issued before any is used (indices past the end are
+    // clamped to the last item, then replaced by pads)
 #pragma unroll
     for (int i = 0; i < IPT; i++) {
         const int e = i * BLOCK + tid;
-        if (e < cnt) {
-            K k;
-            uint64_t vv = 0;
-            GMem<K, HV>::load(in, base + e, k, vv);
-            r[i] = enc_key(k, cin);
-            v[i] = vv;
-            if constexpr (HV) nmax += (r[i] == KMAX);
-        } else {
-            r[i] = KMAX;
-            v[i] = 0;
-        }
+        v[i] = 0;
+        GMem<K, HV>::load(in, base + (e < cnt ? e : cnt - 1), r[i], v[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < IPT; i++) {
+        const bool live = i * BLOCK + tid < cnt;
+        r[i] = live ? enc_key(r[i], cin) : KMAX;
+        v[i] = live ? v[i] : 0;
+        if constexpr (HV) nmax += live && r[i] == KMAX;
     }
     int lim = cnt;
     if constexpr (HV) {
@@ -235,14 +235,26 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
     const int lb = cnt - la;
     const uint64_t A = abase + a0;
     const uint64_t B = bbase + (d0 - a0);
-#pragma unroll 4
-    for (int e = tid; e < cnt; e += BLOCK) {
-        const uint64_t src = e < la ? A + e : B + (e - la);
-        K k;
-        uint64_t vv = 0;
-        GMem<K, HV>::load(in, src, k, vv);
-        sk[lsw<K>(e)] = k;
-        if constexpr (HV) sv[lsw<uint64_t>(e)] = vv;
+    {
+        // all loads in flight at once (indices past the end clamped)
+        K kk[IPT];
+        uint64_t vv[IPT];
+        const uint64_t Bs = B - (uint64_t)la;  // item e >= la is at Bs + e
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int e = i * BLOCK + tid;
+            const int ec = e < cnt ? e : cnt - 1;
+            vv[i] = 0;
+            GMem<K, HV>::load(in, (ec < la ? A : Bs) + (uint64_t)ec, kk[i], vv[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int e = i * BLOCK + tid;
+            if (e < cnt) {
+                sk[lsw<K>(e)] = kk[i];
+                if constexpr (HV) sv[lsw<uint64_t>(e)] = vv[i];
+            }
+        }
     }
     __syncthreads();
     const int d = tid * IPT;
@@ -588,14 +600,27 @@ __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restric
     if (cnt == 0) return;
     const uint64_t sA = q.g4 + a0, sB = q.g4 + q.la + b0, sC = q.g4 + lx + c0, sD = q.g4 + lx + q.lc + d0;
     const int e1 = na, e2 = na + nb, e3 = na + nb + nc;
-#pragma unroll 4
-    for (int e = tid; e < cnt; e += BLOCK) {
-        const uint64_t src = e < e1 ? sA + e : e < e2 ? sB + (e - e1) : e < e3 ? sC + (e - e2) : sD + (e - e3);
-        K key;
-        uint64_t vv = 0;
-        GMem<K, HV>::load(in, src, key, vv);
-        sk[lsw<K>(e)] = key;
-        if constexpr (HV) sv[lsw<uint64_t>(e)] = vv;
+    {
+        K kk[IPT];
+        uint64_t vv[IPT];
+        // item e of the tile is at (base of its slice) + e
+        const uint64_t bA = sA, bB = sB - (uint64_t)e1, bC = sC - (uint64_t)e2, bD = sD - (uint64_t)e3;
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int e0 = i * BLOCK + tid;
+            const int e = e0 < cnt ? e0 : cnt - 1;
+            const uint64_t src = (e < e1 ? bA : e < e2 ? bB : e < e3 ? bC : bD) + (uint64_t)e;
+            vv[i] = 0;
+            GMem<K, HV>::load(in, src, kk[i], vv[i]);
+        }
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int e = i * BLOCK + tid;
+            if (e < cnt) {
+                sk[lsw<K>(e)] = kk[i];
+                if constexpr (HV) sv[lsw<uint64_t>(e)] = vv[i];
+            }
+        }
     }
     __syncthreads();
     const int nx = e2, d = tid * IPT;
