@@ -221,30 +221,16 @@ __device__ __forceinline__ void wave_sort_levels(K (&r)[IPT], uint64_t (&v)[IPT]
 
 // ----------------------------------------------------------------------------
 // Merge path in LDS (the co-rank form of mergebl2: merger.h:964-1169).
-// A = sk[a_off .. a_off+la), B = sk[b_off .. b_off+lb).  Returns the number of
-// A items among the first `diag` outputs; ties go to A (stable).
-// ----------------------------------------------------------------------------
-template <typename K>
-__device__ __forceinline__ int lds_corank(const K *sk, int a_off, int la, int b_off, int lb, int diag) {
-    int lo = diag > lb ? diag - lb : 0;
-    int hi = diag < la ? diag : la;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        K a = sk[lsw<K>(a_off + mid)];
-        K b = sk[lsw<K>(b_off + diag - 1 - mid)];
-        if (!(b < a)) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// The same co-rank by binary lifting: a fixed, wave-uniform number of steps
-// (one per bit of `top`, a power of two with 2*top > min(la, lb)) and no
-// data-dependent branch, so every lane issues the same instructions.  Step s
+// A = sk[a_off .. a_off+la), B = sk[b_off .. b_off+lb).  The co-rank of a
+// diagonal is the number of A items among its first `diag` outputs; ties go
+// to A (stable).  It is found by binary lifting: a fixed, wave-uniform number
+// of steps (one per bit of `top`, a power of two with 2*top > min(la, lb)) and
+// no data-dependent branch, so every lane issues the same instructions.  Step s
 // tries to advance the answer by s (clamped to the upper bound) and keeps the
 // move when the last A item it would take still precedes the B item at the
-// same diagonal.  Reads A[-1] at most (a guard item before the array) and
-// B[lb] at most (slack after the array).
+// same diagonal.  Reads A[-1] at most (guard space before the buffer) and
+// B[lb] at most (guard space after it).
+// ----------------------------------------------------------------------------
 template <typename K>
 __device__ __forceinline__ int lds_corank_lift(const K *sk, int a_off, int la, int b_off, int lb, int diag, int top) {
     const int lo = diag > lb ? diag - lb : 0;
@@ -259,11 +245,13 @@ __device__ __forceinline__ int lds_corank_lift(const K *sk, int a_off, int la, i
     return a;
 }
 
-// Items of LDS guard space around a merge buffer: one 16-byte chunk before it
-// (the co-rank reads A[-1]) and IPT items after it (the bitonic slice merge
-// reads up to IPT-1 items past either run).
-template <typename K> constexpr int lds_front() { return 16 / (int)sizeof(K); }
-template <typename K, int IPT> constexpr int lds_extra() { return lds_front<K>() + IPT; }
+// Items of LDS guard space around a merge buffer: one 128-byte row before it
+// (the co-rank reads A[-1], which the swizzle maps up to one row back) and
+// IPT + 32 items after it (merges read up to IPT-1 items past a run, and the
+// swizzle moves an index by at most 31 within its row).
+template <typename K> constexpr int lds_front() { return 128 / (int)sizeof(K); }
+template <typename K, int IPT> constexpr int lds_back() { return IPT + 32; }
+template <typename K, int IPT> constexpr int lds_extra() { return lds_front<K>() + lds_back<K, IPT>(); }
 
 // Merges this thread's slice of merge(A, B) -- `na` items of A starting at
 // LDS index ia, and IPT - na items of B ending at LDS index jb - na (inclusive)
@@ -297,35 +285,55 @@ __device__ __forceinline__ void lds_merge_bitonic(const K *sk, const uint64_t *s
             if (!(i & s)) cex<HV>(r[i], r[i + s], v[i], v[i + s]);
 }
 
-// Serial branch-free merge of `cnt` outputs into registers r[i0 .. i0+cnt)
-// (i0 + cnt <= IPT) starting at A index ai and B index bi.  Reads one LDS item
-// per output.  `limit` bounds the LDS index read ahead of an exhausted side.
-template <int IPT, bool HV, typename K>
+// Serial branch-free merge of outputs into registers, starting at A index ai
+// and B index bi: all IPT outputs when FULL, else r[i0 .. i0+cnt) (i0 + cnt <=
+// IPT).  Reads one LDS item per output, with no clamp: reading one item past
+// either run stays inside the guard space after the buffer.  Key-only runs
+// read a side that is exhausted as the maximum key, so one compare picks the
+// output (a maximum-key tie then takes the exhausted side, which emits the same
+// bits); KV runs test exhaustion explicitly, so a pad never displaces a value.
+template <int IPT, bool HV, bool FULL, typename K>
 __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv, int a_off, int la, int b_off,
-                                                 int lb, int ai, int bi, int i0, int cnt, int limit, K (&r)[IPT],
+                                                 int lb, int ai, int bi, int i0, int cnt, K (&r)[IPT],
                                                  uint64_t (&v)[IPT]) {
     int ia = a_off + ai, ib = b_off + bi;
     const int ea = a_off + la, eb = b_off + lb;
-    K ka = sk[lsw<K>(min(ia, limit))], kb = sk[lsw<K>(min(ib, limit))];
-    uint64_t va = 0, vb = 0;
-    if constexpr (HV) {
-        va = sv[lsw<uint64_t>(min(ia, limit))];
-        vb = sv[lsw<uint64_t>(min(ib, limit))];
-    }
+    if constexpr (!HV) {
+        constexpr K KMAX = KeyMax<K>::v;
+        K ka = sk[lsw<K>(ia)], kb = sk[lsw<K>(ib)];
+        ka = ia < ea ? ka : KMAX;
+        kb = ib < eb ? kb : KMAX;
 #pragma unroll
-    for (int i = 0; i < IPT; i++) {
-        if (i >= i0 && i < i0 + cnt) {
-            bool takeA = ib >= eb || (ia < ea && !(kb < ka));
-            r[i] = takeA ? ka : kb;
-            if constexpr (HV) v[i] = takeA ? va : vb;
-            ia += takeA;
-            ib += !takeA;
-            int nx = min(takeA ? ia : ib, limit);
-            K nk = sk[lsw<K>(nx)];
-            ka = takeA ? nk : ka;
-            kb = takeA ? kb : nk;
-            if constexpr (HV) {
-                uint64_t nv = sv[lsw<uint64_t>(nx)];
+        for (int i = 0; i < IPT; i++) {
+            if (FULL || (i >= i0 && i < i0 + cnt)) {
+                const bool takeA = !(kb < ka);
+                r[i] = takeA ? ka : kb;
+                ia += takeA;
+                ib += !takeA;
+                const int nx = takeA ? ia : ib;
+                const K nk0 = sk[lsw<K>(nx)];
+                const K nk = nx < (takeA ? ea : eb) ? nk0 : KMAX;
+                ka = takeA ? nk : ka;
+                kb = takeA ? kb : nk;
+                v[i] = 0;
+            }
+        }
+    } else {
+        K ka = sk[lsw<K>(ia)], kb = sk[lsw<K>(ib)];
+        uint64_t va = sv[lsw<uint64_t>(ia)], vb = sv[lsw<uint64_t>(ib)];
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            if (FULL || (i >= i0 && i < i0 + cnt)) {
+                const bool takeA = ib >= eb || (ia < ea && !(kb < ka));
+                r[i] = takeA ? ka : kb;
+                v[i] = takeA ? va : vb;
+                ia += takeA;
+                ib += !takeA;
+                const int nx = takeA ? ia : ib;
+                const K nk = sk[lsw<K>(nx)];
+                const uint64_t nv = sv[lsw<uint64_t>(nx)];
+                ka = takeA ? nk : ka;
+                kb = takeA ? kb : nk;
                 va = takeA ? nv : va;
                 vb = takeA ? vb : nv;
             }
@@ -333,81 +341,18 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
     }
 }
 
-// Merges outputs [d, d + c) (c <= IPT) of merge(A, B) into registers with CH
-// independent chains: chain k starts on its own diagonal d + k*IPT/CH (its own
-// co-rank search), and the chains advance in lockstep, so each step issues CH
-// independent LDS reads instead of one dependent read per output.
-#ifndef OGS_MERGE_CHAINS
-#define OGS_MERGE_CHAINS 1
-#endif
-template <int IPT, int CH, bool HV, typename K>
-__device__ __forceinline__ void lds_merge_chains(const K *sk, const uint64_t *sv, int a_off, int la, int b_off,
-                                                 int lb, int d, int c, int limit, K (&r)[IPT],
-                                                 uint64_t (&v)[IPT]) {
-    static_assert(IPT % CH == 0, "chains must split IPT evenly");
-    constexpr int L = IPT / CH;
-    const int tot = la + lb;
-    int lo[CH], hi[CH], dg[CH];
-#pragma unroll
-    for (int k = 0; k < CH; k++) {
-        dg[k] = min(d + k * L, tot);
-        lo[k] = dg[k] > lb ? dg[k] - lb : 0;
-        hi[k] = dg[k] < la ? dg[k] : la;
-    }
-    bool busy = true;
-    while (busy) {
-        busy = false;
-#pragma unroll
-        for (int k = 0; k < CH; k++) {
-            if (lo[k] < hi[k]) {
-                const int mid = (lo[k] + hi[k]) >> 1;
-                const K a = sk[lsw<K>(a_off + mid)];
-                const K b = sk[lsw<K>(b_off + dg[k] - 1 - mid)];
-                if (!(b < a)) lo[k] = mid + 1;
-                else hi[k] = mid;
-                busy |= lo[k] < hi[k];
-            }
-        }
-    }
-    const int ea = a_off + la, eb = b_off + lb;
-    int ia[CH], ib[CH];
-    K ka[CH], kb[CH];
-    uint64_t va[CH], vb[CH];
-#pragma unroll
-    for (int k = 0; k < CH; k++) {
-        ia[k] = a_off + lo[k];
-        ib[k] = b_off + dg[k] - lo[k];
-        ka[k] = sk[lsw<K>(min(ia[k], limit))];
-        kb[k] = sk[lsw<K>(min(ib[k], limit))];
-        va[k] = vb[k] = 0;
-        if constexpr (HV) {
-            va[k] = sv[lsw<uint64_t>(min(ia[k], limit))];
-            vb[k] = sv[lsw<uint64_t>(min(ib[k], limit))];
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < L; i++) {
-#pragma unroll
-        for (int k = 0; k < CH; k++) {
-            const int o = k * L + i;
-            if (o < c) {
-                const bool takeA = ib[k] >= eb || (ia[k] < ea && !(kb[k] < ka[k]));
-                r[o] = takeA ? ka[k] : kb[k];
-                if constexpr (HV) v[o] = takeA ? va[k] : vb[k];
-                ia[k] += takeA;
-                ib[k] += !takeA;
-                const int nx = min(takeA ? ia[k] : ib[k], limit);
-                const K nk = sk[lsw<K>(nx)];
-                ka[k] = takeA ? nk : ka[k];
-                kb[k] = takeA ? kb[k] : nk;
-                if constexpr (HV) {
-                    const uint64_t nv = sv[lsw<uint64_t>(nx)];
-                    va[k] = takeA ? nv : va[k];
-                    vb[k] = takeA ? vb[k] : nv;
-                }
-            }
-        }
-    }
+// Largest power of two <= x (0 for 0): the co-rank step bound for runs whose
+// shorter side has x items.
+__device__ __forceinline__ int pow2_floor(int x) { return x > 0 ? 1 << (31 - __clz(x)) : 0; }
+
+// Merges outputs [d, d + c) of merge(A, B) into registers: binary-lifting
+// co-rank, then a serial merge (full-slice fast path when c == IPT).
+template <int IPT, bool HV, typename K>
+__device__ __forceinline__ void lds_merge_slice(const K *sk, const uint64_t *sv, int a_off, int la, int b_off,
+                                                int lb, int d, int c, int top, K (&r)[IPT], uint64_t (&v)[IPT]) {
+    const int a = lds_corank_lift<K>(sk, a_off, la, b_off, lb, d, top);
+    if (c == IPT) lds_merge_serial<IPT, HV, true>(sk, sv, a_off, la, b_off, lb, a, d - a, 0, IPT, r, v);
+    else lds_merge_serial<IPT, HV, false>(sk, sv, a_off, la, b_off, lb, a, d - a, 0, c, r, v);
 }
 
 // Store IPT consecutive registers to LDS at item index base (a multiple of

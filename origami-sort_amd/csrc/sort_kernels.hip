@@ -83,7 +83,7 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
         const int ae = diag + IPT == 2 * R ? R : sa[tid + 1];
         lds_merge_bitonic<IPT, HV>(sk, sv, pb + a, ae - a, pb + R + diag - a + IPT - 1, r, v);
 #else
-        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, pb, R, pb + R, R, diag, IPT, TILE - 1, r, v);
+        lds_merge_slice<IPT, HV>(sk, sv, pb, R, pb + R, R, diag, IPT, R, r, v);
 #endif
     }
     __syncthreads();
@@ -96,7 +96,7 @@ template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out, uint64_t n, int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
-    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ int smisc[2];
     __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
@@ -121,7 +121,7 @@ template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const TileDesc *__restrict__ tiles, int cin, int cout) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
-    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ int smisc[2];
     __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
@@ -272,13 +272,12 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
         const int ae = d + IPT == cnt ? la : sa[tid + 1];
         lds_merge_bitonic<IPT, HV>(sk, sv, a, ae - a, la + d - a + IPT - 1, r, v);
     } else if (d < cnt) {
-        lds_merge_serial<IPT, HV>(sk, sv, 0, la, la, lb, a, d - a, 0, cnt - d, cnt - 1, r, v);
+        lds_merge_serial<IPT, HV, false>(sk, sv, 0, la, la, lb, a, d - a, 0, cnt - d, r, v);
     }
 #else
-    if (d < cnt) {
-        const int c = cnt - d < IPT ? cnt - d : IPT;
-        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, 0, la, la, lb, d, c, cnt - 1, r, v);
-    }
+    if (d < cnt)
+        lds_merge_slice<IPT, HV>(sk, sv, 0, la, la, lb, d, cnt - d < IPT ? cnt - d : IPT, pow2_floor(la < lb ? la : lb),
+                                 r, v);
 #endif
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
@@ -291,7 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ i
                                                       uint64_t R, int cout, const uint64_t *__restrict__ part) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
-    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ uint64_t sco[2];
     __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
@@ -336,7 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__res
                                                           const uint64_t *__restrict__ part) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
-    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ uint64_t sco[2];
     __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
@@ -573,7 +572,7 @@ __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restric
     constexpr int TILE = BLOCK * IPT;
     constexpr uint64_t h = TILE / 2;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
-    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE : 2];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     K *sk = skb + lds_front<K>();
     const int tid = threadIdx.x;
     const uint64_t per = quad_slots(R, h);
@@ -629,24 +628,25 @@ __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restric
     if (d < cnt) {
         if (d < nx) {
             const int cx = nx - d < IPT ? nx - d : IPT;
-            const int a = lds_corank<K>(sk, 0, na, na, nb, d);
-            lds_merge_serial<IPT, HV>(sk, sv, 0, na, na, nb, a, d - a, 0, cx, cnt - 1, r, v);
+            const int a = lds_corank_lift<K>(sk, 0, na, na, nb, d, pow2_floor(na < nb ? na : nb));
+            if (cx == IPT) lds_merge_serial<IPT, HV, true>(sk, sv, 0, na, na, nb, a, d - a, 0, IPT, r, v);
+            else lds_merge_serial<IPT, HV, false>(sk, sv, 0, na, na, nb, a, d - a, 0, cx, r, v);
         }
         if (d + IPT > nx && nx < cnt) {
             const int st = d > nx ? d : nx;
             const int yd = st - nx;
             const int cy = (d + IPT < cnt ? d + IPT : cnt) - st;
-            const int c = lds_corank<K>(sk, e2, nc, e3, nd, yd);
-            lds_merge_serial<IPT, HV>(sk, sv, e2, nc, e3, nd, c, yd - c, st - d, cy, cnt - 1, r, v);
+            const int c = lds_corank_lift<K>(sk, e2, nc, e3, nd, yd, pow2_floor(nc < nd ? nc : nd));
+            if (cy == IPT) lds_merge_serial<IPT, HV, true>(sk, sv, e2, nc, e3, nd, c, yd - c, 0, IPT, r, v);
+            else lds_merge_serial<IPT, HV, false>(sk, sv, e2, nc, e3, nd, c, yd - c, st - d, cy, r, v);
         }
     }
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
     __syncthreads();
-    if (d < cnt) {
-        const int c = cnt - d < IPT ? cnt - d : IPT;
-        lds_merge_chains<IPT, OGS_MERGE_CHAINS, HV>(sk, sv, 0, nx, nx, cnt - nx, d, c, cnt - 1, r, v);
-    }
+    if (d < cnt)
+        lds_merge_slice<IPT, HV>(sk, sv, 0, nx, nx, cnt - nx, d, cnt - d < IPT ? cnt - d : IPT,
+                                 pow2_floor(nx < cnt - nx ? nx : cnt - nx), r, v);
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
     __syncthreads();
@@ -680,7 +680,7 @@ static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R,
 // is the default; the tuning table may pick any other.
 #define OGS_BLOCK_SHAPES_32(X) X(256, 16, 1) X(256, 16, 0) X(256, 16, 2) X(256, 16, 3) X(256, 32, 1) \
     X(512, 16, 1) X(512, 32, 1) X(1024, 16, 1) X(1024, 32, 1) X(128, 16, 1) X(256, 8, 1) X(64, 16, 1) X(64, 8, 1) \
-    X(512, 16, 0) X(1024, 16, 0) X(512, 32, 0)
+    X(512, 16, 0) X(1024, 16, 0) X(512, 32, 0) X(512, 32, 2) X(512, 32, 3) X(512, 32, 4) X(512, 16, 3) X(512, 16, 5)
 #define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8) X(1024, 16) \
     X(512, 32) X(128, 4)
 #define OGS_BLOCK_SHAPES_64(X) X(256, 16, 1) X(256, 8, 1) X(128, 16, 1) X(512, 8, 1) X(512, 16, 1) \
