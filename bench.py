@@ -8,7 +8,9 @@ copy runs inside the timed region).  Prints one JSON line:
   value        whole-job G keys/s over the K timed steps
   roofline     the dominant kernel (by summed time) vs the 8 TB/s HBM peak,
                algorithmic bytes per launch (2 * n * w) / its mean launch time,
-               timed with HIP events recorded on the launching stream
+               timed with HIP events recorded on the launching stream during
+               the last timed step (events between every kernel of every step
+               would add a few microseconds per kernel to the whole-job time)
   pass_roofline the pass-count roofline of the whole sort: P * 2 * n * w / t
   cpu_baseline the oracle (a C restatement of Origami's mergesort) on a bounded
                sample of the same input on this host's cores
@@ -278,12 +280,15 @@ def main():
         assert ok_sorted and ok_sum
         del o
 
-    ctx.profile(True)
     ctx.profile_reset()
     torch.cuda.synchronize()
     log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
+        # per-kernel HIP events on the last timed step only: each event adds a
+        # few microseconds between dependent kernels
+        if i == args.steps - 1:
+            ctx.profile(True)
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()

@@ -90,6 +90,7 @@ def tune_class(ctx, name, kind, quick):
     ctx.tune_set(kind, "plain_corank", best[3])
     result["plain_block"], result["plain_merge"], result["plain_corank"] = best[1], best[2], best[3]
     if best[3] == 0:
+        lanes0 = ctx.tune_get(kind, "partition_lanes")  # the width `best` was timed with
         for lanes in (1, 4, 8, 16):
             ctx.tune_set(kind, "partition_lanes", lanes)
             t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
@@ -97,7 +98,18 @@ def tune_class(ctx, name, kind, quick):
             if t < best[0]:
                 best = (t, best[1], best[2], best[3])
                 result["partition_lanes"] = lanes
-        ctx.tune_set(kind, "partition_lanes", result.get("partition_lanes", 4))
+        # keep that width unless another one measured faster
+        result.setdefault("partition_lanes", lanes0)
+        ctx.tune_set(kind, "partition_lanes", result["partition_lanes"])
+    # two merge-tree levels per HBM pass (4-way) against 2-way passes
+    result["plain_fanin"] = 2
+    ctx.tune_set(kind, "plain_fanin", 4)
+    t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
+    log(f"  {name} plain plain_fanin=4: {t * 1e3:.3f} ms")
+    if t < best[0]:
+        best = (t, best[1], best[2], best[3])
+        result["plain_fanin"] = 4
+    ctx.tune_set(kind, "plain_fanin", result["plain_fanin"])
     result["_plain"] = {"n": n, "ms": best[0] * 1e3, "gkeys": n / best[0] / 1e9}
     del d_in, a, b
     # ---- segmented batch: log-uniform lengths in [1, 2^20]

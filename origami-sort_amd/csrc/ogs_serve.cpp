@@ -228,8 +228,21 @@ void Server::schedule_loop() {
                 items_v.push_back({r->src, r->dst, blk, r->n});
             }
         }
-        if (rc == OGS_OK) rc = ogs_sort_batch(reinterpret_cast<ogs_ctx *>(ctx), head->kt, items_v.data(),
-                                              items_v.size(), (void *)s);
+        if (rc == OGS_OK && alone) {
+            // a long request runs as one plain sort: its tile and pass shapes
+            // are the plain ones, and no per-tile descriptors are built
+            const ogs_batch_item &it = items_v[0];
+            void *res = nullptr;
+            rc = ogs_sort_keep(reinterpret_cast<ogs_ctx *>(ctx), head->kt, it.in, it.out, it.scratch, it.n, &res,
+                               (void *)s);
+            if (rc == OGS_OK && res != it.out) {
+                hipError_t ec = hipMemcpyAsync(it.out, res, (size_t)it.n * ogs_item_size(head->kt),
+                                               hipMemcpyDeviceToDevice, s);
+                if (ec != hipSuccess) rc = hip_fail(ec, "result copy");
+            }
+        } else if (rc == OGS_OK) {
+            rc = ogs_sort_batch(reinterpret_cast<ogs_ctx *>(ctx), head->kt, items_v.data(), items_v.size(), (void *)s);
+        }
         hipError_t e = hipEventCreateWithFlags(&b->ev, hipEventDisableTiming);
         if (e == hipSuccess) e = hipEventRecord(b->ev, s);
         if (e != hipSuccess && rc == OGS_OK) rc = hip_fail(e, "batch event");
