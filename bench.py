@@ -63,8 +63,9 @@ def cpu_baseline(kind_oracle, host_keys, sample_n, threads):
 
 def bench_batched(ctx, total_log2, max_seg_log2, seed=4):
     """Batched segmented sort (one ogs_segmented_sort launch sequence over CSR
-    segments with log-uniform lengths in [1, 2^max_seg_log2]) against one plain
-    ogs_sort launch per segment, same u32 keys, both on the GPU."""
+    segments with log-uniform lengths in [1, 2^max_seg_log2], and the same sort
+    captured as a HIP graph and replayed) against one plain ogs_sort launch per
+    segment, same u32 keys, all on the GPU."""
     import numpy as np
     import torch
 
@@ -101,6 +102,22 @@ def bench_batched(ctx, total_log2, max_seg_log2, seed=4):
     t_b /= reps
     res_b = s if p == s.data_ptr() else d
     batched_out = res_b.clone()
+    # the same sort captured once as a HIP graph (ogs_segmented_plan) and
+    # replayed: one hipGraphLaunch per sort
+    plan, pres = ctx.segmented_plan(ogs.U32, d.data_ptr(), s.data_ptr(), doff.data_ptr(), len(lens), n)
+    t_g = 0.0
+    graph_same = True
+    for r in range(reps + 1):
+        d.copy_(d0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.launch()
+        torch.cuda.synchronize()
+        if r:
+            t_g += time.perf_counter() - t0
+        graph_same = graph_same and torch.equal(s if pres == s.data_ptr() else d, batched_out)
+    t_g /= reps
+    plan.close()
     # one plain launch per segment
     d.copy_(d0)
     torch.cuda.synchronize()
@@ -125,8 +142,11 @@ def bench_batched(ctx, total_log2, max_seg_log2, seed=4):
     return {"segments": len(lens), "keys": n, "max_segment": int(max(lens)),
             "lengths": f"log-uniform [1, 2^{max_seg_log2}], drawn until 2^{total_log2} keys",
             "batched_ms": round(t_b * 1e3, 3), "batched_keys_per_s": round(n / t_b / 1e9, 4),
+            "graph_ms": round(t_g * 1e3, 3), "graph_keys_per_s": round(n / t_g / 1e9, 4),
+            "graph_identical": bool(graph_same),
             "plain_per_segment_ms": round(t_p * 1e3, 3), "plain_keys_per_s": round(n / t_p / 1e9, 4),
-            "unit": "G keys/s", "speedup": round(t_p / t_b, 2), "identical_results": bool(same and ok_sorted)}
+            "unit": "G keys/s", "speedup": round(t_p / t_b, 2), "graph_speedup": round(t_p / t_g, 2),
+            "identical_results": bool(same and ok_sorted)}
 
 
 def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5):

@@ -117,6 +117,24 @@ typedef struct {
 } ogs_batch_item;
 int ogs_sort_batch(ogs_ctx *ctx, ogs_key_t kt, const ogs_batch_item *items, uint64_t count, void *stream);
 
+/* A segmented sort captured once as a HIP graph and replayed by one
+ * hipGraphLaunch per call.  It sorts the CSR segments of d_data (as
+ * ogs_segmented_sort: results land in d_scratch, d_data is consumed).
+ * - Bound buffers: the plan holds its own descriptor and co-rank tables. The
+ *   pointers d_data and d_scratch, and the offsets' values, must stay as they
+ *   were at creation. The data's contents may change between launches.
+ * - Every segment runs through the batched kernels; there is no plain-sort
+ *   class.
+ * - Launches of one plan must not overlap.
+ * No reference counterpart (the reference's uniform chunks, sorter.h:3857-3868,
+ * are re-planned on every call). It is for callers that sort one layout
+ * repeatedly, such as a serving loop's fixed request shapes. */
+typedef struct ogs_plan ogs_plan;
+int ogs_segmented_plan(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch, const uint64_t *d_offsets,
+                       uint64_t nseg, uint64_t n, ogs_plan **plan, void **d_sorted);
+int ogs_plan_launch(ogs_plan *plan, void *stream);
+int ogs_plan_destroy(ogs_plan *plan);
+
 /* Host-memory convenience: copies h_data (n items) to the device, sorts, and
  * copies the result back into h_data; synchronous.  This is what
  * origami_gpu_compat.h calls for the reference's host-pointer signatures. */

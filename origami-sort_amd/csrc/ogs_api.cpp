@@ -301,42 +301,44 @@ int Ctx::pass_count(ogs_key_t kt, uint64_t n) {
 //    pass land in `fin` (a segment with P passes starts in fin when P is even).
 // All segments of a class share one launch; all segments' p-th merge pass
 // share one launch.
-int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
-    KeyInfo ki;
-    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
-    hipError_t e = hipSetDevice(device);
-    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_plain, BatchBuild &b) {
+    if (!key_info(kt, &b.ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    const KeyInfo &ki = b.ki;
     const BlockShapeFns &tb = block_shape(ki.kc, TP_SEG_TINY_BLOCK);
     const BlockShapeFns &sb = block_shape(ki.kc, TP_SEG_SMALL_BLOCK);
     const BlockShapeFns &lb = block_shape(ki.kc, TP_SEG_LARGE_BLOCK);
-    const MergeShapeFns &mb = merge_shape(ki.kc, TP_SEG_MERGE, lb.tile());
+    b.mb = &merge_shape(ki.kc, TP_SEG_MERGE, lb.tile());
+    b.cls[0] = &tb;
+    b.cls[1] = &sb;
+    b.cls[2] = &lb;
     uint64_t small_max = (uint64_t)tune[ki.kc].v[TP_SEG_SMALL_MAX];
     if (small_max == 0 || small_max > sb.tile()) small_max = sb.tile();
     uint64_t tiny_max = (uint64_t)tune[ki.kc].v[TP_SEG_TINY_MAX];
     if (tiny_max > tb.tile()) tiny_max = tb.tile();
     if (tiny_max > small_max) tiny_max = small_max;
-    const uint64_t LT = lb.tile(), MT = mb.tile();
+    const uint64_t LT = lb.tile(), MT = b.mb->tile();
+    b.LT = LT;
     const uint64_t w = (uint64_t)ki.width;
-    const uint64_t plain_min = tune[ki.kc].v[TP_SEG_PLAIN_MIN] > 0 ? (uint64_t)1 << tune[ki.kc].v[TP_SEG_PLAIN_MIN]
-                                                                    : UINT64_MAX;
-    const bool seg_table = tune[ki.kc].v[TP_PLAIN_CORANK] == 0;
+    const uint64_t plain_min = allow_plain && tune[ki.kc].v[TP_SEG_PLAIN_MIN] > 0
+                                   ? (uint64_t)1 << tune[ki.kc].v[TP_SEG_PLAIN_MIN]
+                                   : UINT64_MAX;
+    b.seg_table = tune[ki.kc].v[TP_PLAIN_CORANK] == 0;
+    b.lanes = tune[ki.kc].v[TP_PARTITION_LANES];
     auto at = [w](const void *p, uint64_t i) { return (void *)((char *)p + i * w); };
 
-    std::vector<TileDescH> td[3];  // tiny, small, large
     struct Large { size_t i; int passes; };
     std::vector<Large> large;
-    std::vector<size_t> plain;  // segments long enough to run as plain sorts
     for (size_t i = 0; i < nseg; i++) {
         const SegSpec &g = segs[i];
         if (g.len == 0) continue;
         if (!g.in || !g.fin || !g.other || g.fin == g.other)
             return fail(OGS_ERR_ARG, "segment %zu: NULL buffer or fin == other", i);
         if (g.len >= plain_min) {
-            plain.push_back(i);
+            b.plain.push_back(i);
         } else if (g.len <= tiny_max) {
-            td[0].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
+            b.td[0].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
         } else if (g.len <= small_max) {
-            td[1].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
+            b.td[1].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
         } else {
             int p = 0;
             for (uint64_t R = LT; R < g.len; R *= 2) p++;
@@ -344,15 +346,14 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
             void *dst = (p & 1) ? g.other : g.fin;
             const uint32_t fin = p == 0 ? kTileFinalBit : 0u;
             for (uint64_t o = 0; o < g.len; o += LT)
-                td[2].push_back({at(g.in, o), at(dst, o), (uint32_t)std::min<uint64_t>(LT, g.len - o) | fin, 0});
+                b.td[2].push_back({at(g.in, o), at(dst, o), (uint32_t)std::min<uint64_t>(LT, g.len - o) | fin, 0});
         }
     }
-    int max_p = 0;
-    for (auto &L : large) max_p = std::max(max_p, L.passes);
-    std::vector<MergeDescH> desc;
-    std::vector<uint64_t> pass_start(max_p + 1, 0), pass_cnt(max_p + 1, 0);
-    for (int p = 0; p < max_p; p++) {
-        pass_start[p] = desc.size();
+    for (auto &L : large) b.max_p = std::max(b.max_p, L.passes);
+    b.pass_start.assign(b.max_p + 1, 0);
+    b.pass_cnt.assign(b.max_p + 1, 0);
+    for (int p = 0; p < b.max_p; p++) {
+        b.pass_start[p] = b.desc.size();
         for (auto &L : large) {
             if (L.passes <= p) continue;
             const SegSpec &g = segs[L.i];
@@ -361,61 +362,81 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
             const void *src = in_fin ? g.fin : g.other;
             void *dst = in_fin ? g.other : g.fin;
             const uint64_t last = p == L.passes - 1 ? kSegLastBit : 0;
-            for (uint64_t o = 0; o < g.len; o += MT) desc.push_back({src, dst, g.len, o | last});
+            for (uint64_t o = 0; o < g.len; o += MT) b.desc.push_back({src, dst, g.len, o | last});
         }
-        pass_cnt[p] = desc.size() - pass_start[p];
+        b.pass_cnt[p] = b.desc.size() - b.pass_start[p];
+        b.max_pass = std::max<uint64_t>(b.max_pass, b.pass_cnt[p]);
     }
-    // one upload for every descriptor table
-    size_t bytes = 0;
+    // one device table for every descriptor list
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    size_t o_td[3];
+    b.bytes = 0;
     for (int c = 0; c < 3; c++) {
-        o_td[c] = bytes;
-        bytes = align(bytes + td[c].size() * sizeof(TileDescH));
+        b.o_td[c] = b.bytes;
+        b.bytes = align(b.bytes + b.td[c].size() * sizeof(TileDescH));
     }
-    const size_t o_desc = bytes;
-    bytes = align(bytes + desc.size() * sizeof(MergeDescH));
+    b.o_desc = b.bytes;
+    b.bytes = align(b.bytes + b.desc.size() * sizeof(MergeDescH));
+    if (b.desc.empty() && b.td[0].empty() && b.td[1].empty() && b.td[2].empty()) b.bytes = 0;
+    return OGS_OK;
+}
+
+void BatchBuild::pack(uint8_t *h) const {
+    for (int c = 0; c < 3; c++)
+        if (!td[c].empty()) memcpy(h + o_td[c], td[c].data(), td[c].size() * sizeof(TileDescH));
+    if (!desc.empty()) memcpy(h + o_desc, desc.data(), desc.size() * sizeof(MergeDescH));
+}
+
+// Launches a built batch whose descriptor table sits at D (device) and whose
+// co-rank table (max_pass entries, when the batch uses one) is `part`.  Issues
+// kernel launches only, so it can be captured into a graph.
+int Ctx::batch_enqueue(const BatchBuild &b, void *D, uint64_t *part, hipStream_t s) {
+    uint8_t *d = (uint8_t *)D;
+    for (int c = 0; c < 3; c++) {
+        if (b.td[c].empty()) continue;
+        hipError_t e = b.cls[c]->run_seg(d + b.o_td[c], b.td[c].size(), b.ki.codec, b.ki.codec, s);
+        if (e != hipSuccess) return hip_fail(e, "batched tile-sort launch");
+    }
+    for (int p = 0; p < b.max_p; p++) {
+        hipError_t e = b.mb->run_seg(d + b.o_desc + b.pass_start[p] * sizeof(MergeDescH), b.pass_cnt[p], b.LT << p,
+                                     b.ki.codec, b.seg_table ? part : nullptr, b.lanes, s);
+        if (e != hipSuccess) return hip_fail(e, "batched merge launch");
+    }
+    return OGS_OK;
+}
+
+// Batched sort of independent segments; see batch_build for the size classes.
+int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    BatchBuild b;
+    int rc = batch_build(kt, segs, nseg, true, b);
+    if (rc) return rc;
     // plain-class segments first (each a full plain sort on this stream)
-    for (size_t i : plain) {
+    for (size_t i : b.plain) {
         const SegSpec &g = segs[i];
         void *res = nullptr;
-        int rc = sort(kt, g.in, g.fin, g.other, g.len, &res, s);
+        rc = sort(kt, g.in, g.fin, g.other, g.len, &res, s);
         if (rc) return rc;
     }
-    if (bytes == 0) return OGS_OK;
+    if (b.bytes == 0) return OGS_OK;
     Workspace *ws_ = workspace(s);
     std::lock_guard<std::mutex> g(ws_->mu);
-    int rc = ws_->reserve(bytes);
+    rc = ws_->reserve(b.bytes);
     if (rc) return rc;
-    uint64_t max_pass = 0;
-    for (int p = 0; p < max_p; p++) max_pass = std::max<uint64_t>(max_pass, pass_cnt[p]);
-    if (seg_table && max_pass) {
-        rc = ws_->reserve_part(max_pass * sizeof(uint64_t));
+    if (b.seg_table && b.max_pass) {
+        rc = ws_->reserve_part(b.max_pass * sizeof(uint64_t));
         if (rc) return rc;
     }
     // the previous upload from this staging buffer must have been consumed
     e = hipEventSynchronize(ws_->staged);
     if (e != hipSuccess) return hip_fail(e, "staging wait");
     std::vector<uint8_t> &h = ws_->host;
-    h.resize(bytes);
-    for (int c = 0; c < 3; c++)
-        if (!td[c].empty()) memcpy(h.data() + o_td[c], td[c].data(), td[c].size() * sizeof(TileDescH));
-    if (!desc.empty()) memcpy(h.data() + o_desc, desc.data(), desc.size() * sizeof(MergeDescH));
-    e = hipMemcpyAsync(ws_->dev, h.data(), bytes, hipMemcpyHostToDevice, s);
+    h.resize(b.bytes);
+    b.pack(h.data());
+    e = hipMemcpyAsync(ws_->dev, h.data(), b.bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventRecord(ws_->staged, s);
     if (e != hipSuccess) return hip_fail(e, "descriptor upload");
-    uint8_t *D = (uint8_t *)ws_->dev;
-    const BlockShapeFns *cls[3] = {&tb, &sb, &lb};
-    for (int c = 0; c < 3; c++) {
-        e = cls[c]->run_seg(D + o_td[c], td[c].size(), ki.codec, ki.codec, s);
-        if (e != hipSuccess) return hip_fail(e, "batched tile-sort launch");
-    }
-    for (int p = 0; p < max_p; p++) {
-        e = mb.run_seg(D + o_desc + pass_start[p] * sizeof(MergeDescH), pass_cnt[p], LT << p, ki.codec,
-                       seg_table ? (uint64_t *)ws_->part : nullptr, tune[ki.kc].v[TP_PARTITION_LANES], s);
-        if (e != hipSuccess) return hip_fail(e, "batched merge launch");
-    }
-    return OGS_OK;
+    return batch_enqueue(b, ws_->dev, (uint64_t *)ws_->part, s);
 }
 
 // CSR form: segment i = [off[i], off[i+1]) of data; results land in scratch.
@@ -430,17 +451,45 @@ int Ctx::segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_of
         return OGS_OK;
     }
     if (!data || !scratch || data == scratch) return fail(OGS_ERR_ARG, "bad data/scratch buffers");
-    std::vector<SegSpec> segs(nseg);
+    std::vector<SegSpec> segs;
+    int rc = csr_segments(ki, data, scratch, h_off, nseg, segs);
+    if (rc) return rc;
+    *out = scratch;
+    return batch(kt, segs.data(), segs.size(), s);
+}
+
+// CSR offsets -> segments sorted from data into scratch (data is the second
+// buffer).
+int csr_segments(const KeyInfo &ki, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg,
+                 std::vector<SegSpec> &segs) {
+    const uint64_t total = nseg ? h_off[nseg] : 0;
     const uint64_t w = (uint64_t)ki.width;
+    segs.resize(nseg);
     for (uint64_t i = 0; i < nseg; i++) {
         const uint64_t a = h_off[i], b = h_off[i + 1];
         if (b < a || b > total)
             return fail(OGS_ERR_ARG, "offsets not non-decreasing at segment %llu", (unsigned long long)i);
         segs[i] = {(char *)data + a * w, (char *)scratch + a * w, (char *)data + a * w, b - a};
     }
-    *out = scratch;
-    return batch(kt, segs.data(), segs.size(), s);
+    return OGS_OK;
 }
+
+// A captured segmented sort (ogs_segmented_plan).
+struct Plan {
+    int device = 0;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *dev = nullptr;   // descriptor table
+    void *part = nullptr;  // co-rank table
+    ~Plan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (cap) (void)hipStreamDestroy(cap);
+        if (dev) (void)hipFree(dev);
+        if (part) (void)hipFree(part);
+    }
+};
 
 }  // namespace ogs
 
@@ -548,6 +597,94 @@ int ogs_segmented_sort(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch
         return fail(OGS_ERR_ARG, "offsets must start at 0 and end at n=%llu (got %llu..%llu)", (unsigned long long)n,
                     (unsigned long long)off[0], (unsigned long long)off[nseg]);
     return c->segmented(kt, d_data, d_scratch, off.data(), nseg, d_sorted, (hipStream_t)stream);
+}
+
+int ogs_segmented_plan(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch, const uint64_t *d_offsets,
+                       uint64_t nseg, uint64_t n, ogs_plan **plan, void **d_sorted) {
+    if (!ctx || !plan || !d_sorted) return fail(OGS_ERR_ARG, "NULL argument");
+    *plan = nullptr;
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    std::vector<uint64_t> off(nseg + 1, 0);
+    if (nseg) {
+        if (!d_offsets) return fail(OGS_ERR_ARG, "d_offsets is NULL");
+        e = hipMemcpy(off.data(), d_offsets, (nseg + 1) * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "offsets download");
+        if (off[0] != 0 || off[nseg] != n)
+            return fail(OGS_ERR_ARG, "offsets must start at 0 and end at n=%llu (got %llu..%llu)",
+                        (unsigned long long)n, (unsigned long long)off[0], (unsigned long long)off[nseg]);
+    }
+    Plan *p = new Plan();
+    p->device = c->device;
+    *d_sorted = n ? d_scratch : d_data;
+    if (n == 0) {
+        *plan = reinterpret_cast<ogs_plan *>(p);
+        return OGS_OK;
+    }
+    if (!d_data || !d_scratch || d_data == d_scratch) {
+        delete p;
+        return fail(OGS_ERR_ARG, "bad data/scratch buffers");
+    }
+    std::vector<SegSpec> segs;
+    BatchBuild b;
+    int rc = csr_segments(ki, d_data, d_scratch, off.data(), nseg, segs);
+    if (!rc) rc = c->batch_build(kt, segs.data(), segs.size(), false, b);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    auto bail = [&](hipError_t err, const char *what) {
+        delete p;
+        return hip_fail(err, what);
+    };
+    if (b.bytes) {
+        std::vector<uint8_t> h(b.bytes);
+        b.pack(h.data());
+        e = hipMalloc(&p->dev, b.bytes);
+        if (e != hipSuccess) return bail(e, "hipMalloc(plan descriptors)");
+        e = hipMemcpy(p->dev, h.data(), b.bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(e, "plan descriptor upload");
+        if (b.seg_table && b.max_pass) {
+            e = hipMalloc(&p->part, b.max_pass * sizeof(uint64_t));
+            if (e != hipSuccess) return bail(e, "hipMalloc(plan co-rank table)");
+        }
+        e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
+        if (e != hipSuccess) return bail(e, "plan stream");
+        e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return bail(e, "hipStreamBeginCapture");
+        rc = c->batch_enqueue(b, p->dev, (uint64_t *)p->part, p->cap);
+        e = hipStreamEndCapture(p->cap, &p->graph);
+        if (rc) {
+            delete p;
+            return rc;
+        }
+        if (e != hipSuccess) return bail(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) return bail(e, "hipGraphInstantiate");
+    }
+    *plan = reinterpret_cast<ogs_plan *>(p);
+    return OGS_OK;
+}
+
+int ogs_plan_launch(ogs_plan *plan, void *stream) {
+    if (!plan) return fail(OGS_ERR_ARG, "plan is NULL");
+    Plan *p = reinterpret_cast<Plan *>(plan);
+    if (!p->exec) return OGS_OK;
+    hipError_t e = hipSetDevice(p->device);
+    if (e == hipSuccess) e = hipGraphLaunch(p->exec, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    return OGS_OK;
+}
+
+int ogs_plan_destroy(ogs_plan *plan) {
+    if (!plan) return OGS_OK;
+    Plan *p = reinterpret_cast<Plan *>(plan);
+    (void)hipSetDevice(p->device);
+    delete p;
+    return OGS_OK;
 }
 
 int ogs_sort_batch(ogs_ctx *ctx, ogs_key_t kt, const ogs_batch_item *items, uint64_t count, void *stream) {

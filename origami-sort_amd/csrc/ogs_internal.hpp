@@ -95,6 +95,27 @@ struct Profiler {
     void reset();
 };
 
+// The launches of one batched sort, worked out on the host (Ctx::batch_build):
+// tile descriptors per size class (tiny, small, large), merge descriptors per
+// pass, their offsets in one device table of `bytes`, and the segments that
+// run as plain sorts instead.
+struct BatchBuild {
+    KeyInfo ki;
+    const BlockShapeFns *cls[3] = {nullptr, nullptr, nullptr};
+    const MergeShapeFns *mb = nullptr;
+    uint64_t LT = 0;
+    std::vector<TileDescH> td[3];
+    std::vector<MergeDescH> desc;
+    std::vector<uint64_t> pass_start, pass_cnt;
+    int max_p = 0;
+    size_t o_td[3] = {0, 0, 0}, o_desc = 0, bytes = 0;
+    uint64_t max_pass = 0;
+    bool seg_table = true;
+    int lanes = 4;
+    std::vector<size_t> plain;
+    void pack(uint8_t *h) const;  // the device table's bytes, into h[0 .. bytes)
+};
+
 struct Ctx {
     int device = 0;
     Profiler prof;
@@ -114,9 +135,14 @@ struct Ctx {
     int segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg, void **out,
                   hipStream_t s);
     int batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s);
+    // allow_plain: long segments may run as plain sorts (not in a captured plan)
+    int batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_plain, BatchBuild &b);
+    int batch_enqueue(const BatchBuild &b, void *D, uint64_t *part, hipStream_t s);
     int pass_count(ogs_key_t kt, uint64_t n);
 };
 
 void load_default_tiles(ogs_ctx *ctx);
+int csr_segments(const KeyInfo &ki, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg,
+                 std::vector<SegSpec> &segs);
 
 }  // namespace ogs

@@ -61,6 +61,9 @@ def lib():
         L.ogs_segmented_sort.argtypes = [vp, i, vp, vp, vp, u64, u64, ctypes.POINTER(vp), vp]
         L.ogs_sort_host.argtypes = [vp, i, vp, u64, u64]
         L.ogs_sort_batch.argtypes = [vp, i, vp, u64, vp]
+        L.ogs_segmented_plan.argtypes = [vp, i, vp, vp, vp, u64, u64, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+        L.ogs_plan_launch.argtypes = [vp, vp]
+        L.ogs_plan_destroy.argtypes = [vp]
         L.ogs_pass_count.argtypes = [vp, i, u64]
         L.ogs_generate_host.argtypes = [i, i, u64, u64, vp]
         L.ogs_tune_set.argtypes = [vp, i, cp, i]
@@ -103,6 +106,27 @@ def shapes(kind, merge):
         L.ogs_shape_info(kind, merge, idx, ctypes.byref(b), ctypes.byref(p), ctypes.byref(w))
         res.append((b.value, p.value, w.value))
     return res
+
+
+class Plan:
+    """A captured segmented sort (ogs_segmented_plan); launch() replays it."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self, stream=0):
+        check(lib().ogs_plan_launch(self.h, ctypes.c_void_p(stream)))
+
+    def close(self):
+        if self.h is not None:
+            check(lib().ogs_plan_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ServerConfig(ctypes.Structure):
@@ -228,6 +252,15 @@ class Context:
         check(lib().ogs_segmented_sort(self.h, kind, ctypes.c_void_p(data), ctypes.c_void_p(scratch),
                                        ctypes.c_void_p(offsets), nseg, n, ctypes.byref(out), ctypes.c_void_p(stream)))
         return out.value
+
+    def segmented_plan(self, kind, data, scratch, offsets, nseg, n):
+        """ogs_segmented_plan: a graph-captured segmented sort bound to these
+        buffers; returns (Plan, pointer of the result buffer)."""
+        h = ctypes.c_void_p()
+        out = ctypes.c_void_p()
+        check(lib().ogs_segmented_plan(self.h, kind, ctypes.c_void_p(data), ctypes.c_void_p(scratch),
+                                       ctypes.c_void_p(offsets), nseg, n, ctypes.byref(h), ctypes.byref(out)))
+        return Plan(h), out.value
 
     def sort_batch_ptrs(self, kind, items, stream=0):
         """items: list of (in_ptr, out_ptr, scratch_ptr_or_0, n)."""

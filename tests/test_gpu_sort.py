@@ -157,6 +157,53 @@ def test_segmented_sort(ctx, kind):
         assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
 
 
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.KV_U64U64, ogs.F32, ogs.U64])
+def test_segmented_plan_graph_replays(ctx, kind):
+    # the graph-captured segmented sort, replayed several times with eager
+    # work (refilling the input, an eager sort) and synchronisation between
+    # replays; every replay must equal the oracle and the eager result
+    import torch
+    rng = np.random.default_rng(11 + kind)
+    lens = np.concatenate([[0, 1, 2, 33, 64, 1000, 4096, 4097],
+                           rng.integers(0, 70000, 40), [1 << 18, 300001]])
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    n = int(off[-1])
+    doff = torch.from_numpy(off.view(np.int64).copy()).cuda()
+    d = torch.empty(n * ogs.WIDTH[kind], dtype=torch.uint8, device="cuda")
+    s = torch.empty_like(d)
+    plan, res = ctx.segmented_plan(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+    try:
+        assert res == s.data_ptr()
+        for rep in range(3):
+            say(f"case plan kind={ogs.KIND_NAMES[kind]} replay={rep} n={n}")
+            dist = ogs.NORMAL if kind == ogs.F32 else (ogs.ZIPF if rep % 2 else ogs.UNIFORM)
+            a = ogs.generate(kind, dist, n, seed=100 * kind + rep)
+            d.copy_(ogs.to_device(a))
+            torch.cuda.synchronize()
+            plan.launch()
+            torch.cuda.synchronize()
+            got = ogs.from_device(s, kind, n)
+            ref = oracle.segmented_sort(kind, a, off, threads=4)
+            if kind in KV:
+                assert np.array_equal(got["key"], ref["key"])
+                # the same (key, value) records, only reordered within ties
+                oi, oo = np.lexsort((a["value"], a["key"])), np.lexsort((got["value"], got["key"]))
+                assert np.array_equal(a[oi].view(np.uint8), got[oo].view(np.uint8))
+            else:
+                assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+            # eager path on the same buffers between replays
+            d.copy_(ogs.to_device(a))
+            p = ctx.segmented_ptr(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+            torch.cuda.synchronize()
+            eager = ogs.from_device(s if p == s.data_ptr() else d, kind, n)
+            if kind in KV:
+                assert np.array_equal(eager["key"], got["key"])
+            else:
+                assert np.array_equal(eager.view(np.uint8), got.view(np.uint8))
+    finally:
+        plan.close()
+
+
 def test_host_entry_point(ctx):
     say("case ogs_sort_host")
     a = ogs.generate(ogs.I64, ogs.UNIFORM, 123457, seed=9)
