@@ -324,10 +324,9 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
                                    : UINT64_MAX;
     b.seg_table = tune[ki.kc].v[TP_PLAIN_CORANK] == 0;
     b.lanes = tune[ki.kc].v[TP_PARTITION_LANES];
-    auto at = [w](const void *p, uint64_t i) { return (void *)((char *)p + i * w); };
-
-    struct Large { size_t i; int passes; };
-    std::vector<Large> large;
+    (void)w;
+    b.MT = MT;
+    std::vector<size_t> large;  // indices into b.recs
     for (size_t i = 0; i < nseg; i++) {
         const SegSpec &g = segs[i];
         if (g.len == 0) continue;
@@ -335,55 +334,75 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
             return fail(OGS_ERR_ARG, "segment %zu: NULL buffer or fin == other", i);
         if (g.len >= plain_min) {
             b.plain.push_back(i);
-        } else if (g.len <= tiny_max) {
-            b.td[0].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
+            continue;
+        }
+        SegRecH r{g.in, g.fin, g.other, g.len, 0, 0, 0, 0, 0};
+        if (g.len <= tiny_max) {
+            r.cls = 0;
+            r.tidx = b.ntd[0]++;
         } else if (g.len <= small_max) {
-            b.td[1].push_back({g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0});
+            r.cls = 1;
+            r.tidx = b.ntd[1]++;
         } else {
             int p = 0;
             for (uint64_t R = LT; R < g.len; R *= 2) p++;
-            large.push_back({i, p});
-            void *dst = (p & 1) ? g.other : g.fin;
-            const uint32_t fin = p == 0 ? kTileFinalBit : 0u;
-            for (uint64_t o = 0; o < g.len; o += LT)
-                b.td[2].push_back({at(g.in, o), at(dst, o), (uint32_t)std::min<uint64_t>(LT, g.len - o) | fin, 0});
+            r.cls = 2;
+            r.passes = (uint32_t)p;
+            r.tidx = b.ntd[2];
+            r.lidx = (uint32_t)large.size();
+            b.ntd[2] += (g.len + LT - 1) / LT;
+            b.max_p = std::max(b.max_p, p);
+            large.push_back(b.recs.size());
         }
+        b.recs.push_back(r);
     }
-    for (auto &L : large) b.max_p = std::max(b.max_p, L.passes);
+    b.nlarge = large.size();
     b.pass_start.assign(b.max_p + 1, 0);
     b.pass_cnt.assign(b.max_p + 1, 0);
+    b.poff.assign((size_t)b.max_p * b.nlarge, 0);
     for (int p = 0; p < b.max_p; p++) {
-        b.pass_start[p] = b.desc.size();
-        for (auto &L : large) {
-            if (L.passes <= p) continue;
-            const SegSpec &g = segs[L.i];
-            // after p passes the data sits in fin iff (passes - p) is even
-            const bool in_fin = ((L.passes - p) & 1) == 0;
-            const void *src = in_fin ? g.fin : g.other;
-            void *dst = in_fin ? g.other : g.fin;
-            const uint64_t last = p == L.passes - 1 ? kSegLastBit : 0;
-            for (uint64_t o = 0; o < g.len; o += MT) b.desc.push_back({src, dst, g.len, o | last});
+        b.pass_start[p] = b.ndesc;
+        for (size_t k = 0; k < large.size(); k++) {
+            const SegRecH &r = b.recs[large[k]];
+            if ((int)r.passes <= p) continue;
+            b.poff[(size_t)p * b.nlarge + k] = b.ndesc;
+            b.ndesc += (r.len + MT - 1) / MT;
         }
-        b.pass_cnt[p] = b.desc.size() - b.pass_start[p];
+        b.pass_cnt[p] = b.ndesc - b.pass_start[p];
         b.max_pass = std::max<uint64_t>(b.max_pass, b.pass_cnt[p]);
     }
-    // one device table for every descriptor list
+    // one device table: class tables, merge table, then the uploaded part
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     b.bytes = 0;
     for (int c = 0; c < 3; c++) {
         b.o_td[c] = b.bytes;
-        b.bytes = align(b.bytes + b.td[c].size() * sizeof(TileDescH));
+        b.bytes = align(b.bytes + b.ntd[c] * sizeof(TileDescH));
     }
     b.o_desc = b.bytes;
-    b.bytes = align(b.bytes + b.desc.size() * sizeof(MergeDescH));
-    if (b.desc.empty() && b.td[0].empty() && b.td[1].empty() && b.td[2].empty()) b.bytes = 0;
+    b.bytes = align(b.bytes + b.ndesc * sizeof(MergeDescH));
+    b.o_seg = b.bytes;
+    b.bytes = align(b.bytes + b.recs.size() * sizeof(SegRecH));
+    b.o_poff = b.bytes;
+    b.bytes = align(b.bytes + b.poff.size() * sizeof(uint64_t));
+    if (b.recs.empty()) b.bytes = 0;
     return OGS_OK;
 }
 
 void BatchBuild::pack(uint8_t *h) const {
-    for (int c = 0; c < 3; c++)
-        if (!td[c].empty()) memcpy(h + o_td[c], td[c].data(), td[c].size() * sizeof(TileDescH));
-    if (!desc.empty()) memcpy(h + o_desc, desc.data(), desc.size() * sizeof(MergeDescH));
+    if (!recs.empty()) memcpy(h, recs.data(), recs.size() * sizeof(SegRecH));
+    if (!poff.empty()) memcpy(h + (o_poff - o_seg), poff.data(), poff.size() * sizeof(uint64_t));
+}
+
+// Writes the batch's descriptors into the device table at D (whose records
+// and offsets are uploaded).
+int Ctx::batch_expand(const BatchBuild &b, void *D, hipStream_t s) {
+    uint8_t *d = (uint8_t *)D;
+    hipError_t e = launch_expand((const SegRecH *)(d + b.o_seg), b.recs.size(), (TileDescH *)(d + b.o_td[0]),
+                                 (TileDescH *)(d + b.o_td[1]), (TileDescH *)(d + b.o_td[2]),
+                                 (MergeDescH *)(d + b.o_desc), (const uint64_t *)(d + b.o_poff), b.nlarge, b.LT, b.MT,
+                                 (uint32_t)b.ki.width, s);
+    if (e != hipSuccess) return hip_fail(e, "descriptor expansion launch");
+    return OGS_OK;
 }
 
 // Launches a built batch whose descriptor table sits at D (device) and whose
@@ -392,8 +411,8 @@ void BatchBuild::pack(uint8_t *h) const {
 int Ctx::batch_enqueue(const BatchBuild &b, void *D, uint64_t *part, hipStream_t s) {
     uint8_t *d = (uint8_t *)D;
     for (int c = 0; c < 3; c++) {
-        if (b.td[c].empty()) continue;
-        hipError_t e = b.cls[c]->run_seg(d + b.o_td[c], b.td[c].size(), b.ki.codec, b.ki.codec, s);
+        if (!b.ntd[c]) continue;
+        hipError_t e = b.cls[c]->run_seg(d + b.o_td[c], b.ntd[c], b.ki.codec, b.ki.codec, s);
         if (e != hipSuccess) return hip_fail(e, "batched tile-sort launch");
     }
     for (int p = 0; p < b.max_p; p++) {
@@ -431,11 +450,13 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
     e = hipEventSynchronize(ws_->staged);
     if (e != hipSuccess) return hip_fail(e, "staging wait");
     std::vector<uint8_t> &h = ws_->host;
-    h.resize(b.bytes);
+    h.resize(b.bytes - b.o_seg);
     b.pack(h.data());
-    e = hipMemcpyAsync(ws_->dev, h.data(), b.bytes, hipMemcpyHostToDevice, s);
+    e = hipMemcpyAsync((uint8_t *)ws_->dev + b.o_seg, h.data(), b.bytes - b.o_seg, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventRecord(ws_->staged, s);
-    if (e != hipSuccess) return hip_fail(e, "descriptor upload");
+    if (e != hipSuccess) return hip_fail(e, "segment table upload");
+    rc = batch_expand(b, ws_->dev, s);
+    if (rc) return rc;
     return batch_enqueue(b, ws_->dev, (uint64_t *)ws_->part, s);
 }
 
@@ -641,18 +662,26 @@ int ogs_segmented_plan(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch
         return hip_fail(err, what);
     };
     if (b.bytes) {
-        std::vector<uint8_t> h(b.bytes);
+        std::vector<uint8_t> h(b.bytes - b.o_seg);
         b.pack(h.data());
         e = hipMalloc(&p->dev, b.bytes);
         if (e != hipSuccess) return bail(e, "hipMalloc(plan descriptors)");
-        e = hipMemcpy(p->dev, h.data(), b.bytes, hipMemcpyHostToDevice);
-        if (e != hipSuccess) return bail(e, "plan descriptor upload");
+        e = hipMemcpy((uint8_t *)p->dev + b.o_seg, h.data(), b.bytes - b.o_seg, hipMemcpyHostToDevice);
+        if (e != hipSuccess) return bail(e, "plan segment table upload");
         if (b.seg_table && b.max_pass) {
             e = hipMalloc(&p->part, b.max_pass * sizeof(uint64_t));
             if (e != hipSuccess) return bail(e, "hipMalloc(plan co-rank table)");
         }
         e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
         if (e != hipSuccess) return bail(e, "plan stream");
+        // the descriptors are written once, before capture
+        rc = c->batch_expand(b, p->dev, p->cap);
+        if (rc) {
+            delete p;
+            return rc;
+        }
+        e = hipStreamSynchronize(p->cap);
+        if (e != hipSuccess) return bail(e, "plan descriptor expansion");
         e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return bail(e, "hipStreamBeginCapture");
         rc = c->batch_enqueue(b, p->dev, (uint64_t *)p->part, p->cap);

@@ -96,24 +96,30 @@ struct Profiler {
 };
 
 // The launches of one batched sort, worked out on the host (Ctx::batch_build):
-// tile descriptors per size class (tiny, small, large), merge descriptors per
-// pass, their offsets in one device table of `bytes`, and the segments that
-// run as plain sorts instead.
+// descriptor counts per size class (tiny, small, large) and per merge pass,
+// the layout of one device table holding the class tables, the merge table,
+// the per-segment records and the per-pass offsets, and the segments that run
+// as plain sorts instead.  Only the records and offsets are uploaded (`pack`
+// writes them, from byte o_seg on); the descriptor-expansion kernel writes the
+// descriptors on the device.
 struct BatchBuild {
     KeyInfo ki;
     const BlockShapeFns *cls[3] = {nullptr, nullptr, nullptr};
     const MergeShapeFns *mb = nullptr;
-    uint64_t LT = 0;
-    std::vector<TileDescH> td[3];
-    std::vector<MergeDescH> desc;
+    uint64_t LT = 0, MT = 0;
+    uint64_t ntd[3] = {0, 0, 0};
+    uint64_t ndesc = 0;
+    std::vector<SegRecH> recs;
+    std::vector<uint64_t> poff;  // [pass][large segment]
+    uint64_t nlarge = 0;
     std::vector<uint64_t> pass_start, pass_cnt;
     int max_p = 0;
-    size_t o_td[3] = {0, 0, 0}, o_desc = 0, bytes = 0;
+    size_t o_td[3] = {0, 0, 0}, o_desc = 0, o_seg = 0, o_poff = 0, bytes = 0;
     uint64_t max_pass = 0;
     bool seg_table = true;
     int lanes = 4;
     std::vector<size_t> plain;
-    void pack(uint8_t *h) const;  // the device table's bytes, into h[0 .. bytes)
+    void pack(uint8_t *h) const;  // bytes [o_seg, bytes) of the table, into h[0 ..)
 };
 
 struct Ctx {
@@ -137,6 +143,7 @@ struct Ctx {
     int batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s);
     // allow_plain: long segments may run as plain sorts (not in a captured plan)
     int batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_plain, BatchBuild &b);
+    int batch_expand(const BatchBuild &b, void *D, hipStream_t s);
     int batch_enqueue(const BatchBuild &b, void *D, uint64_t *part, hipStream_t s);
     int pass_count(ogs_key_t kt, uint64_t n);
 };

@@ -726,6 +726,52 @@ static const MergeShapeFns kMergeKV[] = {
 
 #define OGS_COUNT(a) ((int)(sizeof(a) / sizeof((a)[0])))
 
+// ----------------------------------------------------------------------------
+// Descriptor expansion of a batched sort: the host uploads one record per
+// segment, and one wavefront per segment writes that segment's tile
+// descriptors (its block-sort tiles, in the buffer that makes its last pass
+// land in `fin`) and its merge descriptors for every pass.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_expand(const SegRecH *__restrict__ segs, TileDescH *td0, TileDescH *td1,
+                                               TileDescH *td2, MergeDescH *md, const uint64_t *__restrict__ poff,
+                                               uint64_t nlarge, uint64_t LT, uint64_t MT, uint32_t w) {
+    const SegRecH g = segs[blockIdx.x];
+    const uint64_t lane = threadIdx.x;
+    if (g.len == 0) return;
+    if (g.cls < 2) {
+        if (lane == 0) (g.cls == 0 ? td0 : td1)[g.tidx] = TileDescH{g.in, g.fin, (uint32_t)g.len | kTileFinalBit, 0};
+        return;
+    }
+    const uint32_t p = g.passes;
+    char *dst = (char *)((p & 1) ? g.other : g.fin);
+    const uint32_t fin = p == 0 ? kTileFinalBit : 0u;
+    const uint64_t nt = (g.len + LT - 1) / LT;
+    for (uint64_t k = lane; k < nt; k += 64) {
+        const uint64_t o = k * LT;
+        const uint64_t c = g.len - o < LT ? g.len - o : LT;
+        td2[g.tidx + k] = TileDescH{(const char *)g.in + o * w, dst + o * w, (uint32_t)c | fin, 0};
+    }
+    const uint64_t nm = (g.len + MT - 1) / MT;
+    for (uint32_t q = 0; q < p; q++) {
+        // after q passes the data sits in fin iff (passes - q) is even
+        const bool in_fin = ((p - q) & 1) == 0;
+        const void *src = in_fin ? g.fin : g.other;
+        void *dd = in_fin ? g.other : g.fin;
+        const uint64_t last = q == p - 1 ? kSegLastBit : 0;
+        const uint64_t m0 = poff[(uint64_t)q * nlarge + g.lidx];
+        for (uint64_t k = lane; k < nm; k += 64) md[m0 + k] = MergeDescH{src, dd, g.len, (k * MT) | last};
+    }
+}
+
+hipError_t launch_expand(const SegRecH *segs, uint64_t nseg, TileDescH *td0, TileDescH *td1, TileDescH *td2,
+                         MergeDescH *md, const uint64_t *poff, uint64_t nlarge, uint64_t LT, uint64_t MT, uint32_t w,
+                         hipStream_t s) {
+    if (nseg == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_expand, dim3((unsigned)nseg), dim3(64), 0, s, segs, td0, td1, td2, md, poff, nlarge, LT, MT,
+                       w);
+    return hipGetLastError();
+}
+
 const BlockShapeFns *block_shapes(KeyClass kc, int *count) {
     switch (kc) {
     case KC_32: *count = OGS_COUNT(kBlock32); return kBlock32;

@@ -26,8 +26,32 @@ struct MergeDescH {
     uint64_t len;
     uint64_t o0;
 };
+static_assert(sizeof(TileDescH) == 24 && sizeof(MergeDescH) == 32, "descriptor layouts are shared with the kernels");
 static constexpr uint64_t kSegLastBit = 1ull << 62;
 static constexpr uint32_t kTileFinalBit = 1u << 30;
+
+// One segment of a batched sort as the descriptor-expansion kernel reads it:
+// its buffers and length, its size class (0 tiny, 1 small, 2 large), its
+// first tile descriptor in that class's table, and for the large class its
+// merge-pass count and its row in the per-pass offset table.
+struct SegRecH {
+    const void *in;
+    void *fin;
+    void *other;
+    uint64_t len;
+    uint64_t tidx;
+    uint32_t cls;
+    uint32_t passes;
+    uint32_t lidx;
+    uint32_t pad;
+};
+// Writes every tile and merge descriptor of a batch on the device (one
+// wavefront per segment): td[c] are the class tables, md the merge table,
+// poff[q * nlarge + k] the first merge descriptor of large segment k in pass
+// q; LT/MT the large-class block and merge tiles, w the item size in bytes.
+hipError_t launch_expand(const SegRecH *segs, uint64_t nseg, TileDescH *td0, TileDescH *td1, TileDescH *td2,
+                         MergeDescH *md, const uint64_t *poff, uint64_t nlarge, uint64_t LT, uint64_t MT, uint32_t w,
+                         hipStream_t s);
 
 typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s);
 typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s);

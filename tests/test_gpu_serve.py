@@ -55,8 +55,11 @@ def test_mixed_requests_same_alone_and_in_mix(ctx):
     for j, (kind, n, a) in enumerate(reqs):
         src = ogs.to_device(a) if n else torch.zeros(16, dtype=torch.uint8, device="cuda")
         dst = src if j % 3 == 0 else torch.empty_like(src)
-        torch.cuda.synchronize()
         bufs.append((src, dst))
+    torch.cuda.synchronize()
+    # one burst of submissions: requests queue behind the busy streams and
+    # are coalesced
+    for (kind, n, a), (src, dst) in zip(reqs, bufs):
         tickets.append(srv.submit(kind, src.data_ptr(), dst.data_ptr(), n))
     for t in tickets:
         srv.wait(t, 60.0)
