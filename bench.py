@@ -32,13 +32,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel):
+def traffic_from_profiles(kernel, workload):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    calibrated against a copy kernel), committed under profiles/."""
+    calibrated against a copy kernel), committed under profiles/; None unless
+    the committed measurement is of this workload."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
+        if d.get("bench_workload") != workload:
+            return None
         return d.get(kernel, {}).get("bytes_per_launch")
     except (OSError, ValueError):
         return None
@@ -54,7 +57,8 @@ def cpu_baseline(kind_oracle, host_keys, sample_n, threads):
         t0 = time.perf_counter()
         out = oracle.sort(kind_oracle, sample, threads=threads, min_k=2)
         times.append(time.perf_counter() - t0)
-    assert bool((out[1:] >= out[:-1]).all())
+    keys = out["key"] if out.dtype.names else out
+    assert bool((keys[1:] >= keys[:-1]).all())
     t = sorted(times)[1]
     return {"value": round(sample_n / t / 1e9, 6), "unit": "G keys/s", "cores": threads, "kind": "port",
             "sample": f"first 2^{sample_n.bit_length() - 1} keys of the same input, median of 3, "
@@ -247,6 +251,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--kind", default="u32", choices=["u32", "f32", "u64", "kv_u64u64"])
+    ap.add_argument("--dist", default=None, choices=["uniform", "zipf", "sorted", "reverse", "distinct16", "normal"],
+                    help="key distribution (default: uniform; normal for f32)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -272,8 +278,11 @@ def main():
     kind = ogs.KINDS[args.kind]
     n = 1 << args.log2n
     w = ogs.WIDTH[kind]
-    dist = ogs.NORMAL if kind == ogs.F32 else ogs.UNIFORM
-    log(f"bench: generating {n} {args.kind} keys (seed 42)")
+    dists = {"uniform": ogs.UNIFORM, "zipf": ogs.ZIPF, "sorted": ogs.SORTED, "reverse": ogs.REVERSE,
+             "distinct16": ogs.DISTINCT16, "normal": ogs.NORMAL}
+    dname = args.dist or ("normal" if kind == ogs.F32 else "uniform")
+    dist = dists[dname]
+    log(f"bench: generating {n} {args.kind} keys, {dname} (seed 42)")
     host = ogs.generate(kind, dist, n, seed=42)
     ctx = ogs.Context(0)
     d_in = ogs.to_device(host)
@@ -327,6 +336,8 @@ def main():
     achieved = bytes_per_launch / (dk["mean_ms"] / 1e3) / 1e9
     P = ctx.pass_count(kind, n)
     pass_gbs = P * 2.0 * n * w / (ms_step / 1e3) / 1e9
+    workload = (f"plain sort, 2^{args.log2n} {args.kind} " + ("key+value pairs" if kind == ogs.KV_U64U64 else "keys")
+                + f", {dname}, ascending" + ("" if kind == ogs.KV_U64U64 else ", key-only"))
     line = {
         "metric": "G keys/s + % HBM roofline, 2^28 u32 sort; batched keys/s and p50/p99 vs plain",
         "value": round(value, 4),
@@ -339,12 +350,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"u32": "u32", "f32": "f32", "u64": "u64", "kv_u64u64": "u64+u64"}[args.kind],
-        "data": "synthetic: raw std::mt19937(42) keys" if kind == ogs.U32 else "synthetic: seeded generator",
-        "config": {"workload": f"plain sort, 2^{args.log2n} {args.kind} keys uniform, ascending, key-only",
-                   "n": n, "item_bytes": w},
+        "data": ("synthetic: raw std::mt19937(42) keys" if kind == ogs.U32 and dname == "uniform"
+                 else f"synthetic: seeded generator, {dname} keys"),
+        "config": {"workload": workload, "n": n, "item_bytes": w},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(dom)},
+                     "traffic": traffic_from_profiles(dom, workload)},
         "pass_roofline": {"P": P, "bytes_per_key": P * 2 * w, "achieved_GBs": round(pass_gbs, 2),
                           "frac": round(pass_gbs / HBM_PEAK_GBS, 4),
                           "single_pass_GBs": round(2.0 * n * w / (ms_step / 1e3) / 1e9, 2)},
