@@ -73,43 +73,40 @@ def tune_class(ctx, name, kind, quick):
     a = torch.empty_like(d_in)
     b = torch.empty_like(d_in)
     best = None
+    # shapes x co-rank source for 2-way passes; shapes for 4-way passes (which
+    # always use the split table)
     for bi, (bb, bp, bw) in enumerate(bs):
         for mi, (mb, mp, _) in enumerate(ms):
-            if mb * mp > 2 * bb * bp or bb * bp < 2048 or mb * mp < 4096:
+            if mb * mp > 2 * bb * bp or bb * bp < 1024 or mb * mp < 2048:
                 continue
-            for corank in (0, 1):
+            for fanin, corank in ((2, 0), (2, 1), (4, 0)):
                 ctx.tune_set(kind, "plain_block", bi)
                 ctx.tune_set(kind, "plain_merge", mi)
                 ctx.tune_set(kind, "plain_corank", corank)
+                ctx.tune_set(kind, "plain_fanin", fanin)
                 t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
-                log(f"  {name} plain n={n} block={bs[bi]} merge={ms[mi]} corank={corank}: {t * 1e3:.3f} ms")
+                log(f"  {name} plain n={n} block={bs[bi]} merge={ms[mi]} fanin={fanin} corank={corank}: "
+                    f"{t * 1e3:.3f} ms")
                 if best is None or t < best[0]:
-                    best = (t, bi, mi, corank)
+                    best = (t, bi, mi, corank, fanin)
     ctx.tune_set(kind, "plain_block", best[1])
     ctx.tune_set(kind, "plain_merge", best[2])
     ctx.tune_set(kind, "plain_corank", best[3])
+    ctx.tune_set(kind, "plain_fanin", best[4])
     result["plain_block"], result["plain_merge"], result["plain_corank"] = best[1], best[2], best[3]
-    if best[3] == 0:
+    result["plain_fanin"] = best[4]
+    if best[3] == 0:  # the partition kernel runs (2-way table or 4-way split points)
         lanes0 = ctx.tune_get(kind, "partition_lanes")  # the width `best` was timed with
         for lanes in (1, 4, 8, 16):
             ctx.tune_set(kind, "partition_lanes", lanes)
             t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
             log(f"  {name} plain partition_lanes={lanes}: {t * 1e3:.3f} ms")
             if t < best[0]:
-                best = (t, best[1], best[2], best[3])
+                best = (t,) + best[1:]
                 result["partition_lanes"] = lanes
         # keep that width unless another one measured faster
         result.setdefault("partition_lanes", lanes0)
         ctx.tune_set(kind, "partition_lanes", result["partition_lanes"])
-    # two merge-tree levels per HBM pass (4-way) against 2-way passes
-    result["plain_fanin"] = 2
-    ctx.tune_set(kind, "plain_fanin", 4)
-    t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
-    log(f"  {name} plain plain_fanin=4: {t * 1e3:.3f} ms")
-    if t < best[0]:
-        best = (t, best[1], best[2], best[3])
-        result["plain_fanin"] = 4
-    ctx.tune_set(kind, "plain_fanin", result["plain_fanin"])
     result["_plain"] = {"n": n, "ms": best[0] * 1e3, "gkeys": n / best[0] / 1e9}
     del d_in, a, b
     # ---- segmented batch: log-uniform lengths in [1, 2^20]

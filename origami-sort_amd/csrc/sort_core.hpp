@@ -319,25 +319,27 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
             }
         }
     } else {
+        // the merge carries each output's LDS position; values are gathered
+        // once afterwards, off the dependency chain of the key merge
         K ka = sk[lsw<K>(ia)], kb = sk[lsw<K>(ib)];
-        uint64_t va = sv[lsw<uint64_t>(ia)], vb = sv[lsw<uint64_t>(ib)];
+        int pos[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
             if (FULL || (i >= i0 && i < i0 + cnt)) {
                 const bool takeA = ib >= eb || (ia < ea && !(kb < ka));
                 r[i] = takeA ? ka : kb;
-                v[i] = takeA ? va : vb;
+                pos[i] = takeA ? ia : ib;
                 ia += takeA;
                 ib += !takeA;
                 const int nx = takeA ? ia : ib;
                 const K nk = sk[lsw<K>(nx)];
-                const uint64_t nv = sv[lsw<uint64_t>(nx)];
                 ka = takeA ? nk : ka;
                 kb = takeA ? kb : nk;
-                va = takeA ? nv : va;
-                vb = takeA ? vb : nv;
             }
         }
+#pragma unroll
+        for (int i = 0; i < IPT; i++)
+            if (FULL || (i >= i0 && i < i0 + cnt)) v[i] = sv[lsw<uint64_t>(pos[i])];
     }
 }
 
