@@ -65,29 +65,44 @@ def cpu_baseline(kind_oracle, host_keys, sample_n, threads):
                       f"oracle/origami_oracle.c sort_multi_thread restatement, {threads} threads, min_k=2"}
 
 
-def bench_batched(ctx, total_log2, max_seg_log2, seed=4):
+def bench_batched(ctx, total_log2, max_seg_log2, seed=4, nseg_fixed=0):
     """Batched segmented sort (one ogs_segmented_sort launch sequence over CSR
     segments with log-uniform lengths in [1, 2^max_seg_log2], and the same sort
     captured as a HIP graph and replayed) against one plain ogs_sort launch per
-    segment, same u32 keys, all on the GPU."""
+    segment, same u32 keys, all on the GPU.  Segments are drawn until
+    2^total_log2 keys, or exactly nseg_fixed of them when that is set (then the
+    keys come from torch's GPU generator: the total is tens of GB)."""
     import numpy as np
     import torch
 
     import ogs
     rng = np.random.default_rng(seed)
-    total_cap = 1 << total_log2
     lens = []
-    acc = 0
-    while acc < total_cap:
-        ln = max(1, int(2.0 ** (rng.random() * max_seg_log2)))
-        ln = min(ln, total_cap - acc)
-        lens.append(ln)
-        acc += ln
+    if nseg_fixed:
+        lens = [max(1, int(2.0 ** (u * max_seg_log2))) for u in rng.random(nseg_fixed)]
+    else:
+        total_cap = 1 << total_log2
+        acc = 0
+        while acc < total_cap:
+            ln = max(1, int(2.0 ** (rng.random() * max_seg_log2)))
+            ln = min(ln, total_cap - acc)
+            lens.append(ln)
+            acc += ln
     off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
     n = int(off[-1])
     log(f"bench: batched {len(lens)} segments, {n} keys")
-    host = ogs.generate(ogs.U32, ogs.UNIFORM, n, seed=seed)
-    d0 = ogs.to_device(host)
+    if nseg_fixed:
+        g = torch.Generator(device="cuda")
+        g.manual_seed(seed)
+        d0 = torch.empty(n * 4, dtype=torch.uint8, device="cuda")
+        d0i = d0.view(torch.int32)
+        for o in range(0, n, 1 << 28):
+            c = min(n, o + (1 << 28)) - o
+            d0i[o:o + c] = torch.randint(-(1 << 31), 1 << 31, (c,), dtype=torch.int64, device="cuda",
+                                         generator=g).to(torch.int32)
+    else:
+        host = ogs.generate(ogs.U32, ogs.UNIFORM, n, seed=seed)
+        d0 = ogs.to_device(host)
     d = torch.empty_like(d0)
     s = torch.empty_like(d0)
     doff = torch.from_numpy(off.view(np.int64).copy()).cuda()
@@ -144,7 +159,8 @@ def bench_batched(ctx, total_log2, max_seg_log2, seed=4):
             break
     ok_sorted = True
     return {"segments": len(lens), "keys": n, "max_segment": int(max(lens)),
-            "lengths": f"log-uniform [1, 2^{max_seg_log2}], drawn until 2^{total_log2} keys",
+            "lengths": (f"log-uniform [1, 2^{max_seg_log2}], {nseg_fixed} segments" if nseg_fixed else
+                        f"log-uniform [1, 2^{max_seg_log2}], drawn until 2^{total_log2} keys"),
             "batched_ms": round(t_b * 1e3, 3), "batched_keys_per_s": round(n / t_b / 1e9, 4),
             "graph_ms": round(t_g * 1e3, 3), "graph_keys_per_s": round(n / t_g / 1e9, 4),
             "graph_identical": bool(graph_same),
@@ -257,6 +273,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--batched-log2-total", type=int, default=30)
+    ap.add_argument("--batched-segments", type=int, default=0,
+                    help="draw exactly this many segments instead (100000 = the literal C4 batch, ~7.6e9 keys)")
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--serving-load", type=float, default=1.5,
@@ -370,7 +388,7 @@ def main():
     if not args.no_batched:
         del d_a, d_b
         torch.cuda.empty_cache()
-        line["batched"] = bench_batched(ctx, args.batched_log2_total, 20)
+        line["batched"] = bench_batched(ctx, args.batched_log2_total, 20, nseg_fixed=args.batched_segments)
     if not args.no_serving:
         torch.cuda.empty_cache()
         line["serving"] = bench_serving(ctx, args.serving_requests, args.serving_load, args.serving_streams)
