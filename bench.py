@@ -169,6 +169,37 @@ def bench_batched(ctx, total_log2, max_seg_log2, seed=4, nseg_fixed=0):
             "identical_results": bool(same and ok_sorted)}
 
 
+def bench_generator(ctx, buf, reps=5):
+    """Device input generation (ogs_generate) into an already allocated
+    buffer: GB/s of keys written, median of `reps`.  The shuffled Pareto
+    bursts include their host burst table and the (tag, key) sort."""
+    import torch
+
+    import ogs
+    res = {}
+    nbytes = buf.numel()
+    cases = [("u32", ogs.U32, ogs.UNIFORM | ogs.GEN_COUNTER), ("u64", ogs.U64, ogs.PARETO),
+             ("u64", ogs.U64, ogs.ZIPF | ogs.GEN_COUNTER), ("u64", ogs.U64, ogs.PARETO_SHUFFLED)]
+    names = {ogs.UNIFORM: "uniform", ogs.PARETO: "pareto", ogs.ZIPF: "zipf", ogs.PARETO_SHUFFLED: "pareto_shuffled"}
+    for kname, kind, dist in cases:
+        w = ogs.WIDTH[kind]
+        n = nbytes // w
+        if dist == ogs.PARETO_SHUFFLED:
+            n = min(n, 1 << 26)  # pairs take 32 bytes per key
+        times = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.generate_ptr(kind, dist, 42, n, buf.data_ptr())
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        t = sorted(times)[reps // 2]
+        res[f"{kname}_{names[dist & 0xFF]}"] = {"n": n, "ms": round(t * 1e3, 3),
+                                                 "GB_per_s": round(n * w / t / 1e9, 1)}
+        log(f"bench: generator {kname} {names[dist & 0xFF]} n={n}: {t * 1e3:.3f} ms")
+    return res
+
+
 def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5):
     """Serving loop: Poisson arrivals of ORDER-BY requests (key type uniform over
     u32/u64/u64+u64, length log-uniform in [2^10, 2^26], keys uniform or Zipf 50/50,
@@ -267,8 +298,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--kind", default="u32", choices=["u32", "f32", "u64", "kv_u64u64"])
-    ap.add_argument("--dist", default=None, choices=["uniform", "zipf", "sorted", "reverse", "distinct16", "normal"],
-                    help="key distribution (default: uniform; normal for f32)")
+    ap.add_argument("--dist", default=None,
+                    choices=["uniform", "zipf", "sorted", "reverse", "distinct16", "normal", "same",
+                             "almost_sorted", "pareto", "pareto_bursts", "pareto_shuffled", "fibonacci"],
+                    help="key distribution (default: uniform; normal for f32); the last six are written "
+                         "on the device by ogs_generate")
+    ap.add_argument("--no-generator", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
@@ -297,13 +332,24 @@ def main():
     n = 1 << args.log2n
     w = ogs.WIDTH[kind]
     dists = {"uniform": ogs.UNIFORM, "zipf": ogs.ZIPF, "sorted": ogs.SORTED, "reverse": ogs.REVERSE,
-             "distinct16": ogs.DISTINCT16, "normal": ogs.NORMAL}
+             "distinct16": ogs.DISTINCT16, "normal": ogs.NORMAL, "same": ogs.SAME,
+             "almost_sorted": ogs.ALMOST_SORTED, "pareto": ogs.PARETO, "pareto_bursts": ogs.PARETO_BURSTS,
+             "pareto_shuffled": ogs.PARETO_SHUFFLED, "fibonacci": ogs.FIBONACCI}
     dname = args.dist or ("normal" if kind == ogs.F32 else "uniform")
     dist = dists[dname]
-    log(f"bench: generating {n} {args.kind} keys, {dname} (seed 42)")
-    host = ogs.generate(kind, dist, n, seed=42)
     ctx = ogs.Context(0)
-    d_in = ogs.to_device(host)
+    sample_n = min(n, 1 << 26)
+    if dist < ogs.SAME:
+        log(f"bench: generating {n} {args.kind} keys, {dname} (seed 42)")
+        host = ogs.generate(kind, dist, n, seed=42)
+        d_in = ogs.to_device(host)
+    else:
+        # counter-based family, written on the device; the host keeps only the
+        # CPU baseline's sample
+        log(f"bench: generating {n} {args.kind} keys on the device, {dname} (seed 42)")
+        d_in = torch.empty(n * w, dtype=torch.uint8, device="cuda")
+        ctx.generate_ptr(kind, dist, 42, n, d_in.data_ptr())
+        host = d_in[: sample_n * w].cpu().numpy().view(ogs.NP_DTYPE[kind])
     d_a = torch.empty_like(d_in)
     d_b = torch.empty_like(d_in)
     torch.cuda.synchronize()
@@ -369,7 +415,8 @@ def main():
         "vs_baseline": None,
         "dtype": {"u32": "u32", "f32": "f32", "u64": "u64", "kv_u64u64": "u64+u64"}[args.kind],
         "data": ("synthetic: raw std::mt19937(42) keys" if kind == ogs.U32 and dname == "uniform"
-                 else f"synthetic: seeded generator, {dname} keys"),
+                 else f"synthetic: seeded generator, {dname} keys"
+                 + (" (counter family, written on the device)" if dist >= ogs.SAME else "")),
         "config": {"workload": workload, "n": n, "item_bytes": w},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
@@ -385,6 +432,8 @@ def main():
             d_b.copy_(d_in)
         torch.cuda.synchronize()
         line["copy_calib_bytes"] = int(d_in.numel())
+    if not args.no_generator:
+        line["generator"] = bench_generator(ctx, d_a)
     if not args.no_batched:
         del d_a, d_b
         torch.cuda.empty_cache()
@@ -397,7 +446,7 @@ def main():
         import oracle
         okind = {"u32": oracle.U32, "f32": oracle.F32, "u64": oracle.U64, "kv_u64u64": oracle.KV_U64U64}[args.kind]
         log(f"bench: cpu baseline on {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(okind, host, min(n, 1 << 26), threads)
+        line["cpu_baseline"] = cpu_baseline(okind, host, sample_n, threads)
     print(json.dumps(line), flush=True)
 
 

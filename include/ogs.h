@@ -144,12 +144,31 @@ int ogs_sort_host(ogs_ctx *ctx, ogs_key_t kt, void *h_data, uint64_t n, uint64_t
  * context's current tiles (the roofline denominator: bytes = P * 2 * n * w). */
 int ogs_pass_count(ogs_ctx *ctx, ogs_key_t kt, uint64_t n);
 
-/* Seeded input generator (the datagen::Writer analogue, Origami/writer.h:47-242,
- * but reproducible): dist 0 uniform (raw std::mt19937 for 4-byte kinds,
- * std::mt19937_64 otherwise), 1 Zipf(s=1.1) over 2^20 ranks, 2 sorted,
- * 3 reverse sorted, 4 sixteen distinct values, 5 N(0,1) float32 with -0.0 at
- * index 0 and +0.0 at n/2.  KV values are the original index.  Host memory. */
+/* Seeded input generators: the datagen::Writer analogue (Origami/writer.h:27-242,
+ * README D1-D9), reproducible.  KV values are the original index.
+ *
+ * Distributions (dist):
+ *  0 uniform                    1 Zipf(s=1.1) over 2^20 ranks
+ *  2 sorted (evenly spaced)     3 reverse sorted
+ *  4 sixteen distinct values    5 N(0,1) float32 bits, -0.0 at 0 and +0.0 at n/2
+ *  6 all the same key (D2)      7 almost sorted: every 7th item the largest key (D5)
+ *  8 Pareto keys min(ceil(7 (1/(1-u) - 1)), 100000) (D6)
+ *  9 Pareto-length bursts of equal keys, back to back (D7)
+ * 10 the same bursts shuffled (D8)
+ * 11 Fibonacci, wrapping past 2^64 (D9; 4-byte kinds keep the low half)
+ *
+ * ogs_generate_host with dist 0-5 draws from std::mt19937 (4-byte kinds,
+ * dist 0) or std::mt19937_64: the parity inputs.
+ * With OGS_GEN_COUNTER | dist, or any dist >= 6, item i is a function of
+ * (seed, i) alone (a splitmix64 hash): that is the family ogs_generate writes
+ * on the device, and ogs_generate_host mirrors it bit for bit. */
+#define OGS_GEN_COUNTER 0x100
 int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h_out);
+/* Device form of the counter family (dist 0-11, OGS_GEN_COUNTER optional),
+ * written to d_out (n items) on `stream` without a host round trip.  Pareto
+ * bursts upload their start table; the shuffled form sorts (tag, key) pairs with
+ * this library's sort in a temporary device buffer. */
+int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *d_out, void *stream);
 
 /* Tile table (the run-time counterpart of the tuned constants in
  * Origami/config.h:45-264).  param: "plain_block", "plain_merge",

@@ -26,6 +26,8 @@ DTYPES = {U32: np.dtype("<u4"), I32: np.dtype("<i4"), U64: np.dtype("<u8"),
           KV_I64I64: KV_I_DTYPE}
 # distributions of orc_generate
 UNIFORM, ZIPF, SORTED, REVERSE, DISTINCT16, NORMAL = range(6)
+SAME, ALMOST_SORTED, PARETO, PARETO_BURSTS, PARETO_SHUFFLED, FIBONACCI = range(6, 12)
+GEN_COUNTER = 0x100
 
 _lib = None
 
@@ -48,6 +50,8 @@ def lib():
         L.orc_kway_merge.argtypes = [i, vp, vp, i, vp, u64]
         L.orc_partition.argtypes = [i, vp, vp, i, u64, vp]
         L.orc_generate.argtypes = [i, i, u64, u64, vp]
+        L.orc_mix64.argtypes = [u64]
+        L.orc_mix64.restype = u64
         L.orc_mt19937.argtypes = [ctypes.c_uint32, vp, u64]
         L.orc_mt19937_64.argtypes = [u64, vp, u64]
         _lib = L
@@ -80,6 +84,10 @@ def generate(kind, dist, n, seed=42):
     rc = lib().orc_generate(kind, dist, seed, n, _p(out))
     assert rc == 0
     return out
+
+
+def mix64(z):
+    return lib().orc_mix64(z)
 
 
 def mt19937(seed, n):

@@ -249,7 +249,10 @@ static void kway_##S(const T *const *runs, const uint64_t *lens, int K, T *out, 
  * sum(S) = rank inside bounds [L,R]; while the largest item left of the split exceeds the   \
  * smallest item right of it, exclude the offending halves and move both pointers by the     \
  * same delta in opposite directions, keeping sum(S) constant.  Here the empty side is      \
- * treated as -inf/+inf and the loop ends on lmax <= rmin (see file header). */              \
+ * treated as -inf/+inf, and equal keys are ordered by run index so the order is total: the  \
+ * loop ends on lmax < rmin at the one split of that order, which keeps the splits of        \
+ * increasing ranks monotone in every run (with ties left open, two searches can split a     \
+ * run of equal keys differently and cut negative-length slices). */                         \
 static void partition_##S(const T *const *runs, const uint64_t *lens, int K, uint64_t rank,   \
                           uint64_t *Sx) {                                                     \
     uint64_t *Lb = (uint64_t *)malloc(K * sizeof(uint64_t));                                  \
@@ -267,10 +270,10 @@ static void partition_##S(const T *const *runs, const uint64_t *lens, int K, uin
         int smax = -1, smin = -1; T lmax, rmin;                                               \
         for (int k = 0; k < K; k++) {                                                         \
             if (Sx[k] > Lb[k]) { T x = runs[k][Sx[k] - 1];                                    \
-                if (smax < 0 || LT(lmax, x)) { lmax = x; smax = k; } }                        \
+                if (smax < 0 || LT(lmax, x) || (!LT(x, lmax) && smax < k)) { lmax = x; smax = k; } }\
             if (Sx[k] < Rb[k]) { T x = runs[k][Sx[k]];                                        \
-                if (smin < 0 || LT(x, rmin)) { rmin = x; smin = k; } } }                      \
-        if (smax < 0 || smin < 0 || !LT(rmin, lmax)) break;                                   \
+                if (smin < 0 || LT(x, rmin) || (!LT(rmin, x) && k < smin)) { rmin = x; smin = k; } } }\
+        if (smax < 0 || smin < 0 || !(LT(rmin, lmax) || (!LT(lmax, rmin) && smin < smax))) break;\
         Rb[smax] = Sx[smax] - 1; Lb[smin] = Sx[smin] + 1;                                     \
         uint64_t dmax = (Rb[smax] - Lb[smax]) / 2, dmin = (Rb[smin] - Lb[smin]) / 2;          \
         uint64_t dl = dmax < dmin ? dmax : dmin;                                              \
@@ -479,14 +482,154 @@ static inline uint64_t mix64(uint64_t z)
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
 }
+uint64_t orc_mix64(uint64_t z) { return mix64(z); }
+
+/* Counter-based family: item i is a function of (seed, i).  The distributions
+ * follow Origami/writer.h:27-242 (README D1-D9): all same (writer.h:142-155),
+ * almost sorted with every 7th item the largest key (:167-173), Pareto keys
+ * min(ceil(beta (1/(1-u) - 1)), cap) with beta = 7 (:73-90; cap 100000),
+ * Pareto-length bursts back to back (:91-120) and shuffled (:133-138, here with
+ * a seeded order), and the wrapping Fibonacci sequence (:175-220, restated as
+ * its loop); u and the keys come from a splitmix64 hash of (seed, stream, i)
+ * instead of the reference's LCG / unseeded engines. */
+static uint64_t o_ctr(uint64_t seed, uint64_t s, uint64_t i)
+{
+    return mix64(mix64(seed ^ (s * 0xD1B54A32D192ED03ull)) + i);
+}
+static double o_unit(uint64_t x) { return (double)(x >> 11) * (1.0 / 9007199254740992.0); }
+static uint64_t o_pareto(uint64_t x)
+{
+    double u = o_unit(x);
+    double f = ceil(7.0 * (1.0 / (1.0 - u) - 1.0));
+    return f > 100000.0 ? 100000u : (uint64_t)f;
+}
+typedef struct { uint64_t tag, key; } o_tagged;
+static int o_tag_cmp(const void *a, const void *b)
+{
+    uint64_t x = ((const o_tagged *)a)->tag, y = ((const o_tagged *)b)->tag;
+    return x < y ? -1 : x > y;
+}
+
+static int orc_generate_counter(int kt, int dist, uint64_t seed, uint64_t n, void *out)
+{
+    int w = width_of(kt);
+    uint64_t *key = (uint64_t *)malloc((n ? n : 1) * sizeof(uint64_t));
+    if (!key) return -1;
+    double *cdf = NULL;
+    const uint64_t R = 1u << 20;
+    if (dist == 1) {
+        cdf = (double *)malloc(R * sizeof(double));
+        double acc = 0;
+        for (uint64_t r = 0; r < R; r++) { acc += pow((double)(r + 1), -1.1); cdf[r] = acc; }
+    }
+    if (dist == 9 || dist == 10) {
+        uint64_t pos = 0, b = 0;
+        while (pos < n) {
+            uint64_t L = o_pareto(o_ctr(seed, 1, b));
+            if (L == 0) L = 1;
+            uint64_t k = o_ctr(seed, 0, b);
+            for (uint64_t j = pos; j < n && j < pos + L; j++) key[j] = k;
+            pos += L;
+            b++;
+        }
+        if (dist == 10 && n) {
+            int bits = 1;
+            while (bits < 64 && (1ull << bits) < n) bits++;
+            uint64_t mask = bits >= 64 ? ~0ull : ((1ull << bits) - 1);
+            o_tagged *t = (o_tagged *)malloc(n * sizeof(o_tagged));
+            for (uint64_t i = 0; i < n; i++) { t[i].tag = (o_ctr(seed, 2, i) & ~mask) | (i & mask); t[i].key = key[i]; }
+            qsort(t, n, sizeof(o_tagged), o_tag_cmp);
+            for (uint64_t i = 0; i < n; i++) key[i] = t[i].key;
+            free(t);
+        }
+    } else if (dist == 11) {
+        /* Origami/writer.h:175-220: 0, 1, then sums until the 64-bit sum wraps,
+         * when the sequence restarts at 0, 1 */
+        uint64_t a = 0, b = 1, c, i = 2;
+        if (n > 0) key[0] = 0;
+        if (n > 1) key[1] = 1;
+        while (i < n) {
+            c = a + b;
+            if (c < b) {
+                a = 0; b = 1;
+                key[i++] = 0;
+                if (i < n) key[i++] = 1;
+            } else {
+                a = b; b = c;
+                key[i++] = b;
+            }
+        }
+    } else {
+        for (uint64_t i = 0; i < n; i++) {
+            switch (dist) {
+            case 0: key[i] = o_ctr(seed, 0, i); break;
+            case 1: {
+                double u = o_unit(o_ctr(seed, 0, i)) * cdf[R - 1];
+                uint64_t lo = 0, hi = R - 1;
+                while (lo < hi) { uint64_t m = (lo + hi) / 2; if (cdf[m] < u) lo = m + 1; else hi = m; }
+                key[i] = mix64(lo);
+            } break;
+            case 2: key[i] = (uint64_t)i * (UINT64_MAX / n); break;
+            case 3: key[i] = (uint64_t)(n - 1 - i) * (UINT64_MAX / n); break;
+            case 4: key[i] = mix64(o_ctr(seed, 0, i) & 15u); break;
+            case 5: {
+                uint64_t sum = 0;
+                for (uint64_t s = 0; s < 3; s++) {
+                    uint64_t x = o_ctr(seed, s, i);
+                    for (int q = 0; q < 4; q++) sum += (x >> (16 * q)) & 0xFFFF;
+                }
+                float f = (float)(((double)sum - 393210.0) / 65536.0);
+                uint32_t bits32; memcpy(&bits32, &f, 4);
+                key[i] = bits32;
+            } break;
+            case 6: key[i] = o_ctr(seed, 0, 0); break;
+            case 7: key[i] = (i % 7 == 0) ? UINT64_MAX : (uint64_t)i * (UINT64_MAX / n); break;
+            case 8: key[i] = o_pareto(o_ctr(seed, 0, i)); break;
+            default: free(key); free(cdf); return -1;
+            }
+        }
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        if (w == 4) {
+            uint32_t k32;
+            switch (dist) {
+            case 2: k32 = (uint32_t)((uint64_t)i * (UINT32_MAX / n)); break;
+            case 3: k32 = (uint32_t)((uint64_t)(n - 1 - i) * (UINT32_MAX / n)); break;
+            case 7: k32 = (i % 7 == 0) ? UINT32_MAX : (uint32_t)((uint64_t)i * (UINT32_MAX / n)); break;
+            case 5: case 8: case 11: k32 = (uint32_t)key[i]; break;
+            default: k32 = (uint32_t)(key[i] >> 32); break;
+            }
+            ((uint32_t *)out)[i] = k32;
+        } else if (w == 8) {
+            ((uint64_t *)out)[i] = key[i];
+        } else {
+            ((kv_t *)out)[i].k = key[i];
+            ((kv_t *)out)[i].v = i;
+        }
+    }
+    if (w == 4 && dist == 5 && n) {
+        ((uint32_t *)out)[0] = 0x80000000u;
+        ((uint32_t *)out)[n / 2] = 0u;
+    }
+    free(key);
+    free(cdf);
+    return 0;
+}
 
 /* dist: 0 uniform, 1 zipf(1.1, 2^20 ranks), 2 sorted, 3 reverse, 4 sixteen
- * distinct, 5 normal(0,1) (f32 only, with -0.0 at 0 and +0.0 at n/2).
- * Writes n items of kind kt; KV values are the original index. */
+ * distinct, 5 normal(0,1) (f32 only, with -0.0 at 0 and +0.0 at n/2), drawn
+ * from mt19937 / mt19937_64; with the 0x100 flag, or for dist 6-11, the
+ * counter-based family above.  Writes n items of kind kt; KV values are the
+ * original index. */
 int orc_generate(int kt, int dist, uint64_t seed, uint64_t n, void *out)
 {
     int w = width_of(kt);
     if (!w) return -1;
+    if ((dist & 0x100) || (dist >= 6 && dist < 12)) {
+        int base = dist & ~0x100;
+        if (base < 0 || base >= 12) return -1;
+        return orc_generate_counter(kt, base, seed, n, out);
+    }
     uint64_t *raw = NULL;
     uint32_t *raw32 = NULL;
     if (w == 4 && dist == 0) {

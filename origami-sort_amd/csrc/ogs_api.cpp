@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/ogs.h"
+#include "gen_core.hpp"
 #include "ogs_internal.hpp"
 #include "sort_kernels.hpp"
 
@@ -714,6 +715,70 @@ int ogs_plan_destroy(ogs_plan *plan) {
     (void)hipSetDevice(p->device);
     delete p;
     return OGS_OK;
+}
+
+int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *d_out, void *stream) {
+    if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    const int d = dist & ~OGS_GEN_COUNTER;
+    if (d < 0 || d >= gen::COUNT) return fail(OGS_ERR_ARG, "unknown distribution %d", dist);
+    if (n == 0) return OGS_OK;
+    if (!d_out) return fail(OGS_ERR_ARG, "d_out is NULL");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    // temporaries: freed once the stream has finished with them
+    std::vector<void *> tmp;
+    auto done = [&](int rc) {
+        if (hipStreamSynchronize(s) != hipSuccess && rc == OGS_OK) rc = hip_fail(hipGetLastError(), "generate");
+        for (void *p : tmp) (void)hipFree(p);
+        return rc;
+    };
+    auto upload = [&](const void *h, size_t bytes, void **dp) {
+        hipError_t err = hipMalloc(dp, bytes);
+        if (err != hipSuccess) return err;
+        tmp.push_back(*dp);
+        return hipMemcpy(*dp, h, bytes, hipMemcpyHostToDevice);
+    };
+    double *d_cdf = nullptr;
+    uint64_t *d_starts = nullptr;
+    uint64_t nb = 0;
+    if (d == gen::ZIPF) {
+        std::vector<double> cdf;
+        zipf_cdf(cdf);
+        e = upload(cdf.data(), cdf.size() * sizeof(double), (void **)&d_cdf);
+        if (e != hipSuccess) return done(hip_fail(e, "Zipf table upload"));
+    }
+    if (d == gen::PARETO_BURSTS || d == gen::PARETO_SHUFFLED) {
+        std::vector<uint64_t> st;
+        burst_starts(seed, n, st);
+        nb = st.size();
+        e = upload(st.data(), st.size() * sizeof(uint64_t), (void **)&d_starts);
+        if (e != hipSuccess) return done(hip_fail(e, "burst table upload"));
+    }
+    if (d != gen::PARETO_SHUFFLED) {
+        e = launch_generate(ki.width, d, seed, n, d_out, d_cdf, d_starts, nb, s);
+        return done(e == hipSuccess ? OGS_OK : hip_fail(e, "generator launch"));
+    }
+    // shuffled bursts: (tag, key) pairs sorted by tag with this library's sort
+    void *p0 = nullptr, *p1 = nullptr;
+    e = hipMalloc(&p0, n * 16);
+    if (e == hipSuccess) {
+        tmp.push_back(p0);
+        e = hipMalloc(&p1, n * 16);
+        if (e == hipSuccess) tmp.push_back(p1);
+    }
+    if (e != hipSuccess) return done(fail(OGS_ERR_NOMEM, "shuffle pairs (%llu B): %s", (unsigned long long)(n * 32),
+                                          hipGetErrorString(e)));
+    e = launch_generate_tagged(seed, n, shuffle_bits(n), d_starts, nb, p0, s);
+    if (e != hipSuccess) return done(hip_fail(e, "generator launch"));
+    void *sorted = nullptr;
+    int rc = c->sort(OGS_KV_U64U64, nullptr, p0, p1, n, &sorted, s);
+    if (rc) return done(rc);
+    e = launch_generate_untag(ki.width, sorted, n, d_out, s);
+    return done(e == hipSuccess ? OGS_OK : hip_fail(e, "generator launch"));
 }
 
 int ogs_sort_batch(ogs_ctx *ctx, ogs_key_t kt, const ogs_batch_item *items, uint64_t count, void *stream) {

@@ -3,6 +3,7 @@
 // the tile-table accessors used by the autotuner.
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -11,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "gen_core.hpp"
 #include "ogs_internal.hpp"
 
 using namespace ogs;
@@ -38,6 +40,84 @@ int param_index(const char *name) {
 
 }  // namespace
 
+namespace ogs {
+
+// Zipf(1.1) cumulative weights over 2^20 ranks, summed in rank order.
+void zipf_cdf(std::vector<double> &cdf) {
+    cdf.resize(gen::kZipfRanks);
+    double acc = 0;
+    for (uint64_t r = 0; r < gen::kZipfRanks; r++) {
+        acc += std::pow((double)(r + 1), -1.1);
+        cdf[r] = acc;
+    }
+}
+
+// First item index of every Pareto burst until n items are covered
+// (burst b: max(1, pareto(ctr(seed, 1, b))) items of key ctr(seed, 0, b)).
+void burst_starts(uint64_t seed, uint64_t n, std::vector<uint64_t> &starts) {
+    starts.clear();
+    for (uint64_t pos = 0, b = 0; pos < n; b++) {
+        starts.push_back(pos);
+        const uint64_t L = gen::pareto(gen::ctr(seed, 1, b));
+        pos += L ? L : 1;
+    }
+}
+
+int shuffle_bits(uint64_t n) {
+    int b = 1;
+    while (b < 64 && (1ull << b) < n) b++;
+    return b;
+}
+
+// Host mirror of the counter-based family (ogs_generate's device output).
+int generate_counter_host(const KeyInfo &ki, int dist, uint64_t seed, uint64_t n, void *out) {
+    const int w = ki.width;
+    std::vector<uint64_t> keys;  // 64-bit keys, for the table-driven distributions
+    if (dist == gen::ZIPF) {
+        std::vector<double> cdf;
+        zipf_cdf(cdf);
+        keys.resize(n);
+        for (uint64_t i = 0; i < n; i++) {
+            const double u = gen::unit(gen::ctr(seed, 0, i)) * cdf[gen::kZipfRanks - 1];
+            keys[i] = gen::mix64((uint64_t)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin()));
+        }
+    } else if (dist == gen::PARETO_BURSTS || dist == gen::PARETO_SHUFFLED) {
+        std::vector<uint64_t> st;
+        burst_starts(seed, n, st);
+        keys.resize(n);
+        for (uint64_t b = 0; b < st.size(); b++) {
+            const uint64_t e = b + 1 < st.size() ? st[b + 1] : n;
+            const uint64_t k = gen::ctr(seed, 0, b);
+            for (uint64_t i = st[b]; i < e; i++) keys[i] = k;
+        }
+        if (dist == gen::PARETO_SHUFFLED) {
+            const int bits = shuffle_bits(n);
+            std::vector<std::pair<uint64_t, uint64_t>> t(n);
+            for (uint64_t i = 0; i < n; i++) t[i] = {gen::shuffle_tag(seed, i, bits), keys[i]};
+            std::sort(t.begin(), t.end());
+            for (uint64_t i = 0; i < n; i++) keys[i] = t[i].second;
+        }
+    }
+    for (uint64_t i = 0; i < n; i++) {
+        const uint64_t k = keys.empty() ? gen::key64(dist, seed, i, n) : keys[i];
+        if (w == 4) {
+            ((uint32_t *)out)[i] = gen::key32(dist, k, i, n);
+        } else if (w == 8) {
+            ((uint64_t *)out)[i] = k;
+        } else {
+            ((uint64_t *)out)[2 * i] = k;
+            ((uint64_t *)out)[2 * i + 1] = i;
+        }
+    }
+    if (w == 4 && dist == gen::NORMAL && n) {
+        ((uint32_t *)out)[0] = 0x80000000u;
+        ((uint32_t *)out)[n / 2] = 0u;
+    }
+    return OGS_OK;
+}
+
+}  // namespace ogs
+
 extern "C" {
 
 // Distributions as documented in include/ogs.h.  4-byte kinds draw raw
@@ -48,8 +128,10 @@ extern "C" {
 int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h_out) {
     KeyInfo ki;
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
-    if (dist < 0 || dist > 5) return fail(OGS_ERR_ARG, "unknown distribution %d", dist);
+    const int base = dist & ~OGS_GEN_COUNTER;
+    if (base < 0 || base >= gen::COUNT) return fail(OGS_ERR_ARG, "unknown distribution %d", dist);
     if (n && !h_out) return fail(OGS_ERR_ARG, "h_out is NULL");
+    if ((dist & OGS_GEN_COUNTER) || base >= gen::SAME) return generate_counter_host(ki, base, seed, n, h_out);
     const int w = ki.width;
     if (w == 4 && dist == 0) {
         std::mt19937 g((uint32_t)seed);

@@ -151,5 +151,10 @@ struct Ctx {
 void load_default_tiles(ogs_ctx *ctx);
 int csr_segments(const KeyInfo &ki, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg,
                  std::vector<SegSpec> &segs);
+// Counter-based generator tables and host mirror (ogs_host.cpp).
+void zipf_cdf(std::vector<double> &cdf);
+void burst_starts(uint64_t seed, uint64_t n, std::vector<uint64_t> &starts);
+int shuffle_bits(uint64_t n);
+int generate_counter_host(const KeyInfo &ki, int dist, uint64_t seed, uint64_t n, void *out);
 
 }  // namespace ogs

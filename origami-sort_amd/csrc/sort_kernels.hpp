@@ -90,6 +90,15 @@ struct MergeShapeFns {
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
 
+// Counter-based input generator (gen_kernels.hip): width 4, 8 or 16 (KV);
+// cdf: the Zipf table (dist 1), starts/nb: Pareto burst starts (dist 9).
+hipError_t launch_generate(int width, int dist, uint64_t seed, uint64_t n, void *out, const double *cdf,
+                           const uint64_t *starts, uint64_t nb, hipStream_t s);
+// Shuffled bursts: (tag, burst key) pairs, then, once sorted by tag, the keys.
+hipError_t launch_generate_tagged(uint64_t seed, uint64_t n, int bits, const uint64_t *starts, uint64_t nb,
+                                  void *pairs, hipStream_t s);
+hipError_t launch_generate_untag(int width, const void *pairs, uint64_t n, void *out, hipStream_t s);
+
 const BlockShapeFns *block_shapes(KeyClass kc, int *count);
 const MergeShapeFns *merge_shapes(KeyClass kc, int *count);
 

@@ -24,6 +24,8 @@ KV_I_DTYPE = np.dtype([("key", "<i8"), ("value", "<i8")])
 NP_DTYPE = {U32: np.dtype("<u4"), I32: np.dtype("<i4"), U64: np.dtype("<u8"), I64: np.dtype("<i8"),
             F32: np.dtype("<f4"), KV_U64U64: KV_DTYPE, KV_I64I64: KV_I_DTYPE}
 UNIFORM, ZIPF, SORTED, REVERSE, DISTINCT16, NORMAL = range(6)
+SAME, ALMOST_SORTED, PARETO, PARETO_BURSTS, PARETO_SHUFFLED, FIBONACCI = range(6, 12)
+GEN_COUNTER = 0x100  # the counter-based family for dists 0-5 (6-11 always are)
 
 _lib = None
 
@@ -66,6 +68,7 @@ def lib():
         L.ogs_plan_destroy.argtypes = [vp]
         L.ogs_pass_count.argtypes = [vp, i, u64]
         L.ogs_generate_host.argtypes = [i, i, u64, u64, vp]
+        L.ogs_generate.argtypes = [vp, i, i, u64, u64, vp, vp]
         L.ogs_tune_set.argtypes = [vp, i, cp, i]
         L.ogs_tune_get.argtypes = [vp, i, cp, ctypes.POINTER(i)]
         L.ogs_tune_load.argtypes = [vp, cp]
@@ -276,6 +279,10 @@ class Context:
     TUNE_PARAMS = ("plain_block", "plain_merge", "plain_corank", "seg_tiny_block", "seg_tiny_max",
                    "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge", "seg_plain_min",
                    "partition_lanes", "plain_fanin")
+
+    def generate_ptr(self, kind, dist, seed, n, out, stream=0):
+        """Counter-based input written on the device (ogs_generate)."""
+        check(lib().ogs_generate(self.h, kind, dist, seed, n, ctypes.c_void_p(out), ctypes.c_void_p(stream)))
 
     def pass_count(self, kind, n):
         return check(lib().ogs_pass_count(self.h, kind, n))

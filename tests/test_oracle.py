@@ -122,7 +122,28 @@ def test_partition_brute_force():
             assert lmax <= rmin
 
 
-KINDS = [oracle.U32, oracle.I32, oracle.F32, oracle.U64, oracle.I64, oracle.KV_U64U64, oracle.KV_I64I64]
+def test_partition_splits_are_monotone_under_ties():
+    # the threads of one merge group cut their slices at the splits of
+    # consecutive ranks, so every run's split must be non-decreasing in rank
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        K = int(rng.integers(2, 6))
+        runs = [np.sort(rng.integers(0, 3, int(rng.integers(1, 50)))).astype(np.uint32) for _ in range(K)]
+        prev = None
+        for rank in range(sum(r.size for r in runs) + 1):
+            s = oracle.partition(oracle.U32, runs, rank).astype(np.int64)
+            assert prev is None or np.all(s >= prev), (rank, prev, s)
+            prev = s
+
+
+@pytest.mark.parametrize("dist", [oracle.SAME, oracle.PARETO_BURSTS, oracle.PARETO])
+def test_threaded_sort_of_bursty_input(dist):
+    a = oracle.generate(oracle.U32, dist, (1 << 20) + 3, seed=42)
+    for threads, min_k in ((8, 2), (4, 4)):
+        assert np.array_equal(oracle.sort(oracle.U32, a, threads=threads, min_k=min_k), np.sort(a))
+
+
+KINDS =[oracle.U32, oracle.I32, oracle.F32, oracle.U64, oracle.I64, oracle.KV_U64U64, oracle.KV_I64I64]
 
 
 @pytest.mark.parametrize("kind", KINDS)
