@@ -179,7 +179,7 @@ int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n
  * a plain sort instead; 0 = never), "partition_lanes" (lanes per merge tile
  * in the partition kernel: 1, 4, 8 or 16), "plain_fanin" (2: one merge level per
  * HBM pass; 4: two levels per pass, the last pass 2-way when one level is
- * left).  Block and merge parameters take an index
+ * left), "quad_lanes" (lanes per tile split in a 4-way pass: 1, 2, 4 or 8).  Block and merge parameters take an index
  * into ogs_shape_info's lists; *_max take an item count.  Segments of up to
  * seg_tiny_max items (0 = no tiny class) get one workgroup of the tiny shape
  * each, segments up to seg_small_max (0 = the small shape's tile) one

@@ -107,6 +107,17 @@ def tune_class(ctx, name, kind, quick):
         # keep that width unless another one measured faster
         result.setdefault("partition_lanes", lanes0)
         ctx.tune_set(kind, "partition_lanes", result["partition_lanes"])
+    if best[4] == 4:  # the 4-way split kernel's lanes per slot
+        qlanes0 = ctx.tune_get(kind, "quad_lanes")
+        for lanes in (1, 2, 4, 8):
+            ctx.tune_set(kind, "quad_lanes", lanes)
+            t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n), reps)
+            log(f"  {name} plain quad_lanes={lanes}: {t * 1e3:.3f} ms")
+            if t < best[0]:
+                best = (t,) + best[1:]
+                result["quad_lanes"] = lanes
+        result.setdefault("quad_lanes", qlanes0)
+        ctx.tune_set(kind, "quad_lanes", result["quad_lanes"])
     result["_plain"] = {"n": n, "ms": best[0] * 1e3, "gkeys": n / best[0] / 1e9}
     del d_in, a, b
     # ---- segmented batch: log-uniform lengths in [1, 2^20]

@@ -69,6 +69,7 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_SEG_PLAIN_MIN] = 22;
     t.v[TP_PARTITION_LANES] = 4;
     t.v[TP_PLAIN_FANIN] = 2;
+    t.v[TP_QUAD_LANES] = 1;
     return t;
 }
 
@@ -261,7 +262,8 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
                 (void)hipEventRecord(ev[0], s);
             }
             if (wy == 4)
-                e = ms.run_quad(src, dst, n, R, last ? ki.codec : 0, part, s, prof.on ? ev[1] : nullptr);
+                e = ms.run_quad(src, dst, n, R, last ? ki.codec : 0, part, tune[ki.kc].v[TP_QUAD_LANES], s,
+                                prof.on ? ev[1] : nullptr);
             else
                 e = ms.run(src, dst, n, R, last ? ki.codec : 0, table ? part : nullptr,
                            tune[ki.kc].v[TP_PARTITION_LANES], s, prof.on ? ev[1] : nullptr);

@@ -29,7 +29,7 @@ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
 const char *kParamNames[TP_COUNT] = {"plain_block",     "plain_merge", "seg_small_block",
                                      "seg_small_max",   "seg_large_block", "seg_merge",
                                      "seg_tiny_block",  "seg_tiny_max", "plain_corank", "seg_plain_min",
-                                     "partition_lanes", "plain_fanin"};
+                                     "partition_lanes", "plain_fanin", "quad_lanes"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -215,6 +215,12 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
     else if (p == TP_PLAIN_CORANK) cnt = 2;
     else if (p == TP_PLAIN_FANIN) {
         if (value != 2 && value != 4) return fail(OGS_ERR_ARG, "plain_fanin must be 2 or 4 (got %d)", value);
+        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
+        return OGS_OK;
+    }
+    else if (p == TP_QUAD_LANES) {
+        if (value != 1 && value != 2 && value != 4 && value != 8)
+            return fail(OGS_ERR_ARG, "quad_lanes must be 1, 2, 4 or 8 (got %d)", value);
         reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
         return OGS_OK;
     }

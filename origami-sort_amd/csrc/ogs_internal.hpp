@@ -40,7 +40,8 @@ enum TuneParam {
     TP_SEG_PLAIN_MIN = 9,   // log2 of the shortest batch segment run as a plain sort (0 = never)
     TP_PARTITION_LANES = 10,// lanes per merge tile in the partition kernel: 1, 4, 8 or 16
     TP_PLAIN_FANIN = 11,    // plain merges: 2 (one level per HBM pass) or 4 (two levels)
-    TP_COUNT = 12
+    TP_QUAD_LANES = 12,     // lanes per tile slot in the 4-way split kernel: 1, 2, 4 or 8
+    TP_COUNT = 13
 };
 struct Tuning {
     int v[TP_COUNT];

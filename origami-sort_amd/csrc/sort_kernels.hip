@@ -70,12 +70,22 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     wave_sort_levels<WLEV, IPT, HV>(r, v, lane);
 #pragma unroll 1
     for (int R = IPT << WLEV; R < TILE; R <<= 1) {
-        __syncthreads();
-        lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
-        __syncthreads();
         const int o = tid * IPT;
         const int pb = o & ~(2 * R - 1);
         const int diag = o - pb;
+        __syncthreads();
+#if OGS_REV_MERGE && !OGS_MERGE_BITONIC
+        if constexpr (!HV) {
+            // odd runs of length R are the B side of this round: descending
+            if (o & R) lds_put_desc<IPT>(sk, 2 * (o & ~(R - 1)) + R - 1 - o, r);
+            else lds_put<IPT, HV>(sk, sv, o, r, v);
+            __syncthreads();
+            lds_merge_slice_rev<IPT>(sk, pb, R, pb + 2 * R - 1, R, diag, IPT, R, r);
+            continue;
+        }
+#endif
+        lds_put<IPT, HV>(sk, sv, o, r, v);
+        __syncthreads();
 #if OGS_MERGE_BITONIC
         const int a = lds_corank_lift<K>(sk, pb, R, pb + R, R, diag, R);
         sa[tid] = a;
@@ -251,8 +261,16 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
         for (int i = 0; i < IPT; i++) {
             const int e = i * BLOCK + tid;
             if (e < cnt) {
-                sk[lsw<K>(e)] = kk[i];
-                if constexpr (HV) sv[lsw<uint64_t>(e)] = vv[i];
+#if OGS_REV_MERGE && !OGS_MERGE_BITONIC
+                // key-only: the B slice descending after A (bitonic layout)
+                if constexpr (!HV) sk[lsw<K>(e < la ? e : cnt - 1 + la - e)] = kk[i];
+#else
+                if constexpr (!HV) sk[lsw<K>(e)] = kk[i];
+#endif
+                if constexpr (HV) {
+                    sk[lsw<K>(e)] = kk[i];
+                    sv[lsw<uint64_t>(e)] = vv[i];
+                }
             }
         }
     }
@@ -260,6 +278,15 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
     const int d = tid * IPT;
     K r[IPT];
     uint64_t v[IPT];
+#if OGS_REV_MERGE && !OGS_MERGE_BITONIC
+    if constexpr (!HV) {
+        if (d < cnt)
+            lds_merge_slice_rev<IPT>(sk, 0, la, cnt - 1, lb, d, cnt - d < IPT ? cnt - d : IPT,
+                                     pow2_floor(la < lb ? la : lb), r);
+#pragma unroll
+        for (int i = 0; i < IPT; i++) v[i] = 0;
+    } else
+#endif
 #if OGS_MERGE_BITONIC
     // One co-rank per thread, exchanged through LDS so each thread knows where
     // its slice ends; the one thread whose slice is partial merges serially.
@@ -438,9 +465,9 @@ static hipError_t launch_seg_merge(const void *tiles, uint64_t ntiles, uint64_t 
 // A, B, C, D), the output is merge(X, Y).
 //
 // Tiles are cut at points of the (X, Y) merge path: one at every h-th item of
-// X, one at every h-th item of Y (h = TILE/2), and the group start.  Between
-// consecutive points X and Y each advance by at most h, so a tile holds at
-// most TILE items, and every point splits all four runs exactly:
+// X, one at every h-th item of Y (h = BLOCK*IPT/2), and the group start.
+// Between consecutive points X and Y each advance by at most h, so an interval
+// holds at most 2h items, and every point splits all four runs exactly:
 //  - the point at X index x: (a, b) = co-rank of (A, B) at x, v = X[x], and Y
 //    contributes its items below v: c = #C < v, d = #D < v;
 //  - the point at Y index y: (c, d) = co-rank of (C, D) at y, w = Y[y], and X
@@ -524,10 +551,10 @@ __device__ __forceinline__ void rank_pair(const void *in, uint64_t pbase, uint64
     rq = lo1;
 }
 
-// One thread per tile slot: writes the slot's split (a, b, c, d) of its group
+// One thread per tile slot (the form with the fewest loads): writes the slot's split (a, b, c, d) of its group
 // to tab[4*slot ..].
 template <typename K, bool HV>
-__global__ void k_quad_bounds(const void *__restrict__ in, uint64_t n, uint64_t R, uint64_t h, uint64_t nslots,
+__global__ void k_quad_bounds1(const void *__restrict__ in, uint64_t n, uint64_t R, uint64_t h, uint64_t nslots,
                               uint64_t *__restrict__ tab) {
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nslots) return;
@@ -560,97 +587,211 @@ __global__ void k_quad_bounds(const void *__restrict__ in, uint64_t n, uint64_t 
     t[1] = make_ulonglong2(c, d);
 }
 
-// One workgroup per tile slot: loads the tile's four slices into LDS, merges
-// A+B and C+D into one LDS array (level 1: each thread's IPT outputs may span
-// the X/Y boundary), merges that X and Y (level 2), and writes the tile with
-// lane-consecutive stores.  Slots past the end of a ragged group, and empty
-// tiles, exit at once.
+// Items of the sorted run P[0, len) below v (upper = false) or up to v
+// (upper = true), found by the LANES lanes of one group with a (LANES+1)-ary
+// search: every step probes LANES evenly spaced items and keeps the gap in
+// which the predicate turns true.
+template <typename K, bool HV, int LANES>
+__device__ __forceinline__ uint64_t group_bound(const void *in, uint64_t base, uint64_t len, K v, bool upper, int l,
+                                                int gshift) {
+    uint64_t lo = 0, hi = len;
+    while (hi - lo > (uint64_t)LANES) {
+        const uint64_t span = hi - lo;
+        const uint64_t m = lo + (span * (uint64_t)(l + 1)) / (LANES + 1);
+        const K k = GMem<K, HV>::load_key(in, base + m);
+        const bool p = upper ? v < k : !(k < v);
+        const uint64_t bits = (__ballot(p) >> gshift) & ((1ull << LANES) - 1);
+        if (!bits) {
+            lo = lo + (span * LANES) / (LANES + 1) + 1;
+        } else {
+            const int f = __ffsll((unsigned long long)bits) - 1;
+            const uint64_t mf = lo + (span * (uint64_t)(f + 1)) / (LANES + 1);
+            lo = f ? lo + (span * (uint64_t)f) / (LANES + 1) + 1 : lo;
+            hi = mf;
+        }
+    }
+    bool p = false;
+    if ((uint64_t)l < hi - lo) {
+        const K k = GMem<K, HV>::load_key(in, base + lo + l);
+        p = upper ? v < k : !(k < v);
+    }
+    const uint64_t bits = (__ballot(p) >> gshift) & ((1ull << LANES) - 1);
+    return bits ? lo + (uint64_t)(__ffsll((unsigned long long)bits) - 1) : hi;
+}
+
+// LANES lanes per tile slot: write the slot's split (a, b, c, d) of its group
+// to tab[4*slot ..].  The 2-way co-rank uses all LANES lanes; the two ranks in
+// the other pair of runs use LANES/2 lanes each, side by side.  Groups past
+// the end search a valid point and write nothing, so every lane of a wave
+// takes part in every ballot of its group.
+template <typename K, bool HV, int LANES>
+__global__ void k_quad_bounds(const void *__restrict__ in, uint64_t n, uint64_t R, uint64_t h, uint64_t nslots,
+                              uint64_t *__restrict__ tab) {
+    static_assert(LANES >= 2 && LANES <= 64 && (LANES & (LANES - 1)) == 0, "LANES: a power of two");
+    constexpr int HL = LANES / 2;
+    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = (int)(gt % LANES);
+    const int gshift = (int)(threadIdx.x & 63) & ~(LANES - 1);
+    uint64_t s = gt / LANES;
+    bool live = s < nslots;
+    if (!live) s = 0;
+    const uint64_t per = quad_slots(R, h);
+    const uint64_t g = s / per, k = s % per;
+    const QuadGeom q = quad_geom(g * 4 * R, n, R);
+    const uint64_t lx = q.la + q.lb, ly = q.lc + q.ld;
+    const uint64_t A = q.g4, B = A + q.la, C = A + lx, D = C + q.lc;
+    const uint64_t half = 2 * R / h;
+    const bool isx = k > 0 && k - 1 < half;
+    const uint64_t idx = k == 0 ? 0 : (isx ? k - 1 : k - 1 - half) * h;
+    if (k == 0 || idx >= (isx ? lx : ly)) {
+        // the group start, or a point past a ragged group's end
+        if (live && k == 0 && l == 0) {
+            ulonglong2 *t = (ulonglong2 *)(tab + 4 * s);
+            t[0] = make_ulonglong2(0, 0);
+            t[1] = make_ulonglong2(0, 0);
+        }
+        return;
+    }
+    // the split's own pair (P, Q) and the other pair (U, V)
+    const uint64_t P = isx ? A : C, lp = isx ? q.la : q.lc, Q = isx ? B : D, lq = isx ? q.lb : q.ld;
+    const uint64_t U = isx ? C : A, lu = isx ? q.lc : q.la, V = isx ? D : B, lv = isx ? q.ld : q.lb;
+    const uint64_t p = group_corank<K, HV, LANES>(in, P, lp, Q, lq, idx, l, gshift);
+    const K w = next_merged<K, HV>(in, P, lp, Q, lq, p, idx - p);
+    // X points count Y's items below X[x]; Y points count X's items up to Y[y]
+    const bool second = l >= HL;
+    const uint64_t r = group_bound<K, HV, HL>(in, second ? V : U, second ? lv : lu, w, !isx, l - (second ? HL : 0),
+                                              gshift + (second ? HL : 0));
+    const int base = (int)(threadIdx.x & 63) & ~(LANES - 1);
+    const uint64_t ru = __shfl(r, base), rv = __shfl(r, base + HL);
+    if (!live || l != 0) return;
+    const uint64_t slot = 1 + (idx / h) + (ru + rv + h - 1) / h;
+    ulonglong2 *t = (ulonglong2 *)(tab + 4 * (g * per + slot));
+    if (isx) {
+        t[0] = make_ulonglong2(p, idx - p);
+        t[1] = make_ulonglong2(ru, rv);
+    } else {
+        t[0] = make_ulonglong2(ru, rv);
+        t[1] = make_ulonglong2(p, idx - p);
+    }
+}
+
+// Level 1 for the outputs [d, d + IPT) of the concatenation [X | Y] of one
+// tile, X = merge(A, B) at LDS [0, na + nb) and Y = merge(C, D) after it: a
+// thread whose outputs straddle the X/Y boundary merges both parts.
+template <int IPT, bool HV, typename K>
+__device__ __forceinline__ void quad_level1(const K *sk, const uint64_t *sv, int na, int nb, int nc, int nd, int d,
+                                            K (&r)[IPT], uint64_t (&v)[IPT]) {
+    const int nx = na + nb, cnt = nx + nc + nd;
+    if (d < nx) {
+        const int cx = nx - d < IPT ? nx - d : IPT;
+        const int a = lds_corank_lift<K>(sk, 0, na, na, nb, d, pow2_floor(na < nb ? na : nb));
+        if (cx == IPT) lds_merge_serial<IPT, HV, true>(sk, sv, 0, na, na, nb, a, d - a, 0, IPT, r, v);
+        else lds_merge_serial<IPT, HV, false>(sk, sv, 0, na, na, nb, a, d - a, 0, cx, r, v);
+    }
+    if (d + IPT > nx && nx < cnt) {
+        const int st = d > nx ? d : nx;
+        const int yd = st - nx;
+        const int cy = (d + IPT < cnt ? d + IPT : cnt) - st;
+        const int e3 = nx + nc;
+        const int c = lds_corank_lift<K>(sk, nx, nc, e3, nd, yd, pow2_floor(nc < nd ? nc : nd));
+        if (cy == IPT) lds_merge_serial<IPT, HV, true>(sk, sv, nx, nc, e3, nd, c, yd - c, 0, IPT, r, v);
+        else lds_merge_serial<IPT, HV, false>(sk, sv, nx, nc, e3, nd, c, yd - c, st - d, cy, r, v);
+    }
+}
+
+// One workgroup per two consecutive tile slots of a group (the intervals from
+// point 2w to point 2w + 2).  Along the merge path the X and Y points come in
+// near pairs, so one interval is often almost empty and two consecutive ones
+// hold close to BLOCK * IPT items; each interval holds at most 2h = BLOCK * IPT,
+// so the pair fits the 2 * BLOCK * IPT items of LDS.  The workgroup loads the
+// four slices into LDS, merges A+B and C+D (level 1), merges the two results
+// (level 2), and writes the tile.  Each level covers the tile in one or two
+// rounds of BLOCK * IPT outputs; both rounds are merged into registers before
+// either is written back, since the merge reads what the write overwrites.
+// Slots past the end of a ragged group, and empty tiles, exit at once.
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restrict__ in, void *__restrict__ out,
                                                            uint64_t n, uint64_t R, int cout,
                                                            const uint64_t *__restrict__ tab) {
-    constexpr int TILE = BLOCK * IPT;
-    constexpr uint64_t h = TILE / 2;
-    __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
-    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
+    constexpr int C = BLOCK * IPT;  // outputs per round
+    constexpr int CAP = 2 * C;      // LDS items: two intervals of at most C
+    constexpr uint64_t h = C / 2;
+    __shared__ __attribute__((aligned(16))) K skb[CAP + lds_extra<K, IPT>()];
+    __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? CAP + lds_back<K, IPT>() : 2];
     K *sk = skb + lds_front<K>();
     const int tid = threadIdx.x;
     const uint64_t per = quad_slots(R, h);
-    const uint64_t s = blockIdx.x, g = s / per, k = s % per;
+    const uint64_t wpg = (per + 1) / 2;
+    const uint64_t g = blockIdx.x / wpg, s0 = 2 * (blockIdx.x % wpg);
     const QuadGeom q = quad_geom(g * 4 * R, n, R);
     const uint64_t lx = q.la + q.lb, ly = q.lc + q.ld;
     const uint64_t used = 1 + (lx + h - 1) / h + (ly + h - 1) / h;
-    if (k >= used) return;
-    const uint64_t *t0 = tab + 4 * s;
+    if (s0 >= used) return;
+    const uint64_t *t0 = tab + 4 * (g * per + s0);
     const uint64_t a0 = t0[0], b0 = t0[1], c0 = t0[2], d0 = t0[3];
     uint64_t a1 = q.la, b1 = q.lb, c1 = q.lc, d1 = q.ld;
-    if (k + 1 < used) {
-        a1 = t0[4];
-        b1 = t0[5];
-        c1 = t0[6];
-        d1 = t0[7];
+    if (s0 + 2 < used) {
+        a1 = t0[8];
+        b1 = t0[9];
+        c1 = t0[10];
+        d1 = t0[11];
     }
     // a split table that is not monotone or overfills the tile is never read
     // past: such a tile is skipped (its output stays unwritten)
     if (a0 > a1 || a1 > q.la || b0 > b1 || b1 > q.lb || c0 > c1 || c1 > q.lc || d0 > d1 || d1 > q.ld) return;
-    if ((a1 - a0) + (b1 - b0) + (c1 - c0) + (d1 - d0) > (uint64_t)TILE) return;
+    if ((a1 - a0) + (b1 - b0) + (c1 - c0) + (d1 - d0) > (uint64_t)CAP) return;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0), nc = (int)(c1 - c0), nd = (int)(d1 - d0);
     const int cnt = na + nb + nc + nd;
     if (cnt == 0) return;
     const uint64_t sA = q.g4 + a0, sB = q.g4 + q.la + b0, sC = q.g4 + lx + c0, sD = q.g4 + lx + q.lc + d0;
     const int e1 = na, e2 = na + nb, e3 = na + nb + nc;
     {
-        K kk[IPT];
-        uint64_t vv[IPT];
         // item e of the tile is at (base of its slice) + e
         const uint64_t bA = sA, bB = sB - (uint64_t)e1, bC = sC - (uint64_t)e2, bD = sD - (uint64_t)e3;
 #pragma unroll
-        for (int i = 0; i < IPT; i++) {
-            const int e0 = i * BLOCK + tid;
-            const int e = e0 < cnt ? e0 : cnt - 1;
-            const uint64_t src = (e < e1 ? bA : e < e2 ? bB : e < e3 ? bC : bD) + (uint64_t)e;
-            vv[i] = 0;
-            GMem<K, HV>::load(in, src, kk[i], vv[i]);
-        }
+        for (int half = 0; half < 2; half++) {
+            if (half * C >= cnt) break;
+            K kk[IPT];
+            uint64_t vv[IPT];
 #pragma unroll
-        for (int i = 0; i < IPT; i++) {
-            const int e = i * BLOCK + tid;
-            if (e < cnt) {
-                sk[lsw<K>(e)] = kk[i];
-                if constexpr (HV) sv[lsw<uint64_t>(e)] = vv[i];
+            for (int i = 0; i < IPT; i++) {
+                const int e0 = half * C + i * BLOCK + tid;
+                const int e = e0 < cnt ? e0 : cnt - 1;
+                const uint64_t src = (e < e1 ? bA : e < e2 ? bB : e < e3 ? bC : bD) + (uint64_t)e;
+                vv[i] = 0;
+                GMem<K, HV>::load(in, src, kk[i], vv[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < IPT; i++) {
+                const int e = half * C + i * BLOCK + tid;
+                if (e < cnt) {
+                    sk[lsw<K>(e)] = kk[i];
+                    if constexpr (HV) sv[lsw<uint64_t>(e)] = vv[i];
+                }
             }
         }
     }
     __syncthreads();
-    const int nx = e2, d = tid * IPT;
-    K r[IPT];
-    uint64_t v[IPT];
-    if (d < cnt) {
-        if (d < nx) {
-            const int cx = nx - d < IPT ? nx - d : IPT;
-            const int a = lds_corank_lift<K>(sk, 0, na, na, nb, d, pow2_floor(na < nb ? na : nb));
-            if (cx == IPT) lds_merge_serial<IPT, HV, true>(sk, sv, 0, na, na, nb, a, d - a, 0, IPT, r, v);
-            else lds_merge_serial<IPT, HV, false>(sk, sv, 0, na, na, nb, a, d - a, 0, cx, r, v);
-        }
-        if (d + IPT > nx && nx < cnt) {
-            const int st = d > nx ? d : nx;
-            const int yd = st - nx;
-            const int cy = (d + IPT < cnt ? d + IPT : cnt) - st;
-            const int c = lds_corank_lift<K>(sk, e2, nc, e3, nd, yd, pow2_floor(nc < nd ? nc : nd));
-            if (cy == IPT) lds_merge_serial<IPT, HV, true>(sk, sv, e2, nc, e3, nd, c, yd - c, 0, IPT, r, v);
-            else lds_merge_serial<IPT, HV, false>(sk, sv, e2, nc, e3, nd, c, yd - c, st - d, cy, r, v);
-        }
-    }
+    const bool two = cnt > C;  // uniform over the workgroup
+    const int dA = tid * IPT, dB = C + tid * IPT;
+    K r0[IPT], r1[IPT];
+    uint64_t v0[IPT], v1[IPT];
+    if (dA < cnt) quad_level1<IPT, HV>(sk, sv, na, nb, nc, nd, dA, r0, v0);
+    if (two && dB < cnt) quad_level1<IPT, HV>(sk, sv, na, nb, nc, nd, dB, r1, v1);
     __syncthreads();
-    if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
+    if (dA < cnt) lds_put<IPT, HV>(sk, sv, dA, r0, v0);
+    if (two && dB < cnt) lds_put<IPT, HV>(sk, sv, dB, r1, v1);
     __syncthreads();
-    if (d < cnt)
-        lds_merge_slice<IPT, HV>(sk, sv, 0, nx, nx, cnt - nx, d, cnt - d < IPT ? cnt - d : IPT,
-                                 pow2_floor(nx < cnt - nx ? nx : cnt - nx), r, v);
+    const int nx = e2, ny = cnt - e2;
+    const int top = pow2_floor(nx < ny ? nx : ny);
+    if (dA < cnt) lds_merge_slice<IPT, HV>(sk, sv, 0, nx, nx, ny, dA, cnt - dA < IPT ? cnt - dA : IPT, top, r0, v0);
+    if (two && dB < cnt)
+        lds_merge_slice<IPT, HV>(sk, sv, 0, nx, nx, ny, dB, cnt - dB < IPT ? cnt - dB : IPT, top, r1, v1);
     __syncthreads();
-    if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
+    if (dA < cnt) lds_put<IPT, HV>(sk, sv, dA, r0, v0);
+    if (two && dB < cnt) lds_put<IPT, HV>(sk, sv, dB, r1, v1);
     __syncthreads();
-    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, q.g4 + a0 + b0 + c0 + d0, cnt, cout);
+    lds_store_tile<BLOCK, CAP, HV>(sk, sv, out, q.g4 + a0 + b0 + c0 + d0, cnt, cout);
 }
 
 // Slots of one 4-way pass over n items with runs of R and this tile size.
@@ -659,19 +800,33 @@ __host__ __device__ inline uint64_t quad_pass_slots(uint64_t n, uint64_t R, uint
 }
 
 // Launches the two kernels of one 4-way pass; tab holds 4 * quad_pass_slots
-// entries.  `mid` is recorded between them.
+// entries.  The split kernel runs with quad_lanes lanes per slot (1: one
+// thread per slot, the fewest loads; 2, 4, 8: fewer dependent steps, more
+// loads).  `mid` is recorded between the two.
 template <typename K, bool HV, int BLOCK, int IPT>
 static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
-                              hipStream_t s, hipEvent_t mid) {
+                              int quad_lanes, hipStream_t s, hipEvent_t mid) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     if (n == 0) return hipSuccess;
     if (R % (TILE / 2)) return hipErrorInvalidValue;
     const uint64_t nslots = quad_pass_slots(n, R, TILE);
-    hipLaunchKernelGGL((k_quad_bounds<K, HV>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, in, n, R,
-                       TILE / 2, nslots, tab);
+    const uint64_t per = quad_slots(R, TILE / 2);
+    const uint64_t nwg = (n + 4 * R - 1) / (4 * R) * ((per + 1) / 2);
+    if (quad_lanes <= 1)
+        hipLaunchKernelGGL((k_quad_bounds1<K, HV>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, in, n, R,
+                           TILE / 2, nslots, tab);
+    else if (quad_lanes == 2)
+        hipLaunchKernelGGL((k_quad_bounds<K, HV, 2>), dim3((unsigned)((nslots * 2 + 255) / 256)), dim3(256), 0, s, in,
+                           n, R, TILE / 2, nslots, tab);
+    else if (quad_lanes == 4)
+        hipLaunchKernelGGL((k_quad_bounds<K, HV, 4>), dim3((unsigned)((nslots * 4 + 255) / 256)), dim3(256), 0, s, in,
+                           n, R, TILE / 2, nslots, tab);
+    else
+        hipLaunchKernelGGL((k_quad_bounds<K, HV, 8>), dim3((unsigned)((nslots * 8 + 255) / 256)), dim3(256), 0, s, in,
+                           n, R, TILE / 2, nslots, tab);
     if (mid) (void)hipEventRecord(mid, s);
-    hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)nslots), dim3(BLOCK), 0, s, in, out, n,
-                       R, cout, (const uint64_t *)tab);
+    hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)nwg), dim3(BLOCK), 0, s, in, out, n, R,
+                       cout, (const uint64_t *)tab);
     return hipGetLastError();
 }
 

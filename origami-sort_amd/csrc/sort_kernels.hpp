@@ -76,7 +76,7 @@ struct BlockShapeFns {
 // tab (4 * quad_table_slots entries) is filled, then the merge runs; `mid` is
 // recorded between the two.
 typedef hipError_t (*QuadLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
-                                 hipStream_t s, hipEvent_t mid);
+                                 int quad_lanes, hipStream_t s, hipEvent_t mid);
 // Tile slots of one 4-way pass (groups of 4R, 1 + 8R/tile slots each).
 inline uint64_t quad_table_slots(uint64_t n, uint64_t R, uint64_t tile) {
     return (n + 4 * R - 1) / (4 * R) * (1 + 8 * R / tile);

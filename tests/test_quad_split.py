@@ -9,7 +9,9 @@ relies on, for random runs with heavy ties, sorted and reverse-ordered runs,
 and ragged last groups:
   - the slots are a bijection onto [0, 1 + ceil(lx/h) + ceil(ly/h));
   - consecutive points are monotone in all four runs, at most h apart in X and
-    in Y (so a tile holds at most 2h items);
+    in Y (so an interval holds at most 2h items, and the two intervals a
+    workgroup of k_quad_merge_pass takes, from point 2w to point 2w + 2, fit its
+    4h-item LDS tile);
   - merging each tile's four slices (A, B, C, D tie order) and concatenating
     the tiles gives the 4-way merge of the group.
 The merge tree it is checked against is the reference's order of merging
@@ -89,6 +91,8 @@ def check_group(A, B, C, D, h):
         nx = (s1[0] - s0[0]) + (s1[1] - s0[1])
         ny = (s1[2] - s0[2]) + (s1[3] - s0[3])
         assert nx <= h and ny <= h, (nx, ny, h)
+    for s0, s2 in zip(slots[:-1:2], slots[2::2] + [slots[-1]]):
+        assert sum(s2) - sum(s0) <= 4 * h
         X = merge2(A[s0[0]:s1[0]], B[s0[1]:s1[1]])
         Y = merge2(C[s0[2]:s1[2]], D[s0[3]:s1[3]])
         out += merge2(X, Y)
