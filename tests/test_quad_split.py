@@ -91,11 +91,12 @@ def check_group(A, B, C, D, h):
         nx = (s1[0] - s0[0]) + (s1[1] - s0[1])
         ny = (s1[2] - s0[2]) + (s1[3] - s0[3])
         assert nx <= h and ny <= h, (nx, ny, h)
-    for s0, s2 in zip(slots[:-1:2], slots[2::2] + [slots[-1]]):
-        assert sum(s2) - sum(s0) <= 4 * h
         X = merge2(A[s0[0]:s1[0]], B[s0[1]:s1[1]])
         Y = merge2(C[s0[2]:s1[2]], D[s0[3]:s1[3]])
         out += merge2(X, Y)
+    # a workgroup's two intervals: from point 2w to point 2w + 2 (or the end)
+    for p0, p2 in zip(slots[:-1:2], slots[2::2] + [slots[-1]]):
+        assert sum(p2) - sum(p0) <= 4 * h
     assert out == merge2(merge2(A, B), merge2(C, D))
 
 
