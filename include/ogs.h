@@ -88,6 +88,17 @@ int ogs_sort(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch, uint64_t
 int ogs_sort_keep(ogs_ctx *ctx, ogs_key_t kt, const void *d_in, void *d_data, void *d_scratch,
                   uint64_t n, void **d_sorted, void *stream);
 
+/* Key-value sort of a structure of arrays: n keys in d_keys and n values in
+ * d_values, 8 bytes each (kt: OGS_KV_U64U64 or OGS_KV_I64I64).  d_work is
+ * 32 * n bytes of device scratch.  The pairs are packed into the 16-byte
+ * records of KeyValue<i64,i64> (Origami/commons.h:51-72) in d_work, sorted
+ * there, and the sorted keys and values written back to d_keys and d_values,
+ * all ordered on `stream`: two passes more than an AoS sort.  The reference
+ * has no such form (its KV type is AoS only); this is the layout adapter
+ * for callers that hold keys and values apart. */
+int ogs_sort_soa(ogs_ctx *ctx, ogs_key_t kt, void *d_keys, void *d_values, void *d_work, uint64_t n,
+                 void *stream);
+
 /* Sorts every chunk of `chunk` items independently (the last may be shorter):
  * the reference's sort_single_thread with sort_n < n (Origami/sorter.h:3857-3868,
  * and the chunked benchmark Origami/bench_sort_phases.cpp:1307-1425). */

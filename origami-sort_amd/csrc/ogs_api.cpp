@@ -207,7 +207,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     // levels of the merge tree per HBM pass) while at least two levels remain
     // and the table asks for fan-in 4, else 2-way; start on the buffer that
     // makes the last pass land in `data` when reading from a separate input
-    const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4;
+    const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4 && ms.quad_ok;
     std::vector<int> ways;
     for (uint64_t R = T; R < n;) {
         const int wy = (quad && 2 * R < n) ? 4 : 2;
@@ -288,7 +288,7 @@ int Ctx::pass_count(ogs_key_t kt, uint64_t n) {
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
     if (n == 0) return 0;
     const uint64_t T = block_shape(ki.kc, TP_PLAIN_BLOCK).tile();
-    const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4;
+    const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4 && merge_shape(ki.kc, TP_PLAIN_MERGE, T).quad_ok;
     int p = 1;
     for (uint64_t R = T; R < n; p++) R *= (quad && 2 * R < n) ? 4 : 2;
     return p;
@@ -562,6 +562,29 @@ int ogs_sort_keep(ogs_ctx *ctx, ogs_key_t kt, const void *d_in, void *d_data, vo
     if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
     if (!d_in && n) return fail(OGS_ERR_ARG, "d_in is NULL");
     return reinterpret_cast<Ctx *>(ctx)->sort(kt, d_in, d_data, d_scratch, n, d_sorted, (hipStream_t)stream);
+}
+
+int ogs_sort_soa(ogs_ctx *ctx, ogs_key_t kt, void *d_keys, void *d_values, void *d_work, uint64_t n,
+                 void *stream) {
+    if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    if (ki.width != 16) return fail(OGS_ERR_ARG, "ogs_sort_soa takes a key-value type (got %d)", (int)kt);
+    if (n == 0) return OGS_OK;
+    if (!d_keys || !d_values || !d_work) return fail(OGS_ERR_ARG, "d_keys, d_values and d_work must be set");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    char *wa = (char *)d_work, *wb = wa + 16 * n;
+    e = launch_pack_soa(d_keys, d_values, wa, n, s);
+    if (e != hipSuccess) return hip_fail(e, "pack launch");
+    void *res = nullptr;
+    int rc = c->sort(kt, nullptr, wa, wb, n, &res, s);
+    if (rc) return rc;
+    e = launch_unpack_soa(res, d_keys, d_values, n, s);
+    if (e != hipSuccess) return hip_fail(e, "unpack launch");
+    return OGS_OK;
 }
 
 int ogs_profile_enable(ogs_ctx *ctx, int on) {

@@ -807,27 +807,34 @@ template <typename K, bool HV, int BLOCK, int IPT>
 static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
                               int quad_lanes, hipStream_t s, hipEvent_t mid) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
-    if (n == 0) return hipSuccess;
-    if (R % (TILE / 2)) return hipErrorInvalidValue;
-    const uint64_t nslots = quad_pass_slots(n, R, TILE);
-    const uint64_t per = quad_slots(R, TILE / 2);
-    const uint64_t nwg = (n + 4 * R - 1) / (4 * R) * ((per + 1) / 2);
-    if (quad_lanes <= 1)
-        hipLaunchKernelGGL((k_quad_bounds1<K, HV>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, in, n, R,
-                           TILE / 2, nslots, tab);
-    else if (quad_lanes == 2)
-        hipLaunchKernelGGL((k_quad_bounds<K, HV, 2>), dim3((unsigned)((nslots * 2 + 255) / 256)), dim3(256), 0, s, in,
-                           n, R, TILE / 2, nslots, tab);
-    else if (quad_lanes == 4)
-        hipLaunchKernelGGL((k_quad_bounds<K, HV, 4>), dim3((unsigned)((nslots * 4 + 255) / 256)), dim3(256), 0, s, in,
-                           n, R, TILE / 2, nslots, tab);
-    else
-        hipLaunchKernelGGL((k_quad_bounds<K, HV, 8>), dim3((unsigned)((nslots * 8 + 255) / 256)), dim3(256), 0, s, in,
-                           n, R, TILE / 2, nslots, tab);
-    if (mid) (void)hipEventRecord(mid, s);
-    hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)nwg), dim3(BLOCK), 0, s, in, out, n, R,
-                       cout, (const uint64_t *)tab);
-    return hipGetLastError();
+    // the 4-way tile holds 2 * TILE items (keys, and values for KV) in LDS
+    constexpr uint64_t lds = 2 * TILE * (sizeof(K) + (HV ? sizeof(uint64_t) : 0));
+    if constexpr (lds > 160 * 1024) {
+        (void)in, (void)out, (void)n, (void)R, (void)cout, (void)tab, (void)quad_lanes, (void)s, (void)mid;
+        return hipErrorInvalidConfiguration;  // this merge shape has no 4-way form
+    } else {
+        if (n == 0) return hipSuccess;
+        if (R % (TILE / 2)) return hipErrorInvalidValue;
+        const uint64_t nslots = quad_pass_slots(n, R, TILE);
+        const uint64_t per = quad_slots(R, TILE / 2);
+        const uint64_t nwg = (n + 4 * R - 1) / (4 * R) * ((per + 1) / 2);
+        if (quad_lanes <= 1)
+            hipLaunchKernelGGL((k_quad_bounds1<K, HV>), dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s, in, n, R,
+                               TILE / 2, nslots, tab);
+        else if (quad_lanes == 2)
+            hipLaunchKernelGGL((k_quad_bounds<K, HV, 2>), dim3((unsigned)((nslots * 2 + 255) / 256)), dim3(256), 0, s, in,
+                               n, R, TILE / 2, nslots, tab);
+        else if (quad_lanes == 4)
+            hipLaunchKernelGGL((k_quad_bounds<K, HV, 4>), dim3((unsigned)((nslots * 4 + 255) / 256)), dim3(256), 0, s, in,
+                               n, R, TILE / 2, nslots, tab);
+        else
+            hipLaunchKernelGGL((k_quad_bounds<K, HV, 8>), dim3((unsigned)((nslots * 8 + 255) / 256)), dim3(256), 0, s, in,
+                               n, R, TILE / 2, nslots, tab);
+        if (mid) (void)hipEventRecord(mid, s);
+        hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)nwg), dim3(BLOCK), 0, s, in, out, n, R,
+                           cout, (const uint64_t *)tab);
+        return hipGetLastError();
+    }
 }
 
 // Registry of compiled shapes per key class.  Block-sort shapes are
@@ -846,7 +853,9 @@ static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R,
 #define OGS_MERGE_SHAPES_KV(X) X(256, 8) X(128, 8) X(256, 4) X(512, 4) X(512, 8) X(1024, 4) X(128, 4)
 
 #define OGS_BX(K, HV, B, I, W) {B, I, W, &launch_block<K, HV, B, I, W>, &launch_seg_block<K, HV, B, I, W>},
-#define OGS_MX(K, HV, B, I) {B, I, &launch_merge<K, HV, B, I>, &launch_seg_merge<K, HV, B, I>, &launch_quad<K, HV, B, I>},
+#define OGS_MX(K, HV, B, I)                                                                                   \
+    {B, I, &launch_merge<K, HV, B, I>, &launch_seg_merge<K, HV, B, I>, &launch_quad<K, HV, B, I>,              \
+     2ull * B * I * (sizeof(K) + (HV ? sizeof(uint64_t) : 0)) <= 160 * 1024},
 
 static const BlockShapeFns kBlock32[] = {
 #define X(B, I, W) OGS_BX(uint32_t, false, B, I, W)

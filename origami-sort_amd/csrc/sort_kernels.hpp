@@ -87,8 +87,14 @@ struct MergeShapeFns {
     MergeLaunch run;
     SegMergeLaunch run_seg;
     QuadLaunch run_quad;
+    bool quad_ok;  // the 4-way tile (2 * tile items) fits in LDS
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
+
+// Structure-of-arrays key-value layout (layout_kernels.hip): keys[i], vals[i]
+// <-> 16-byte {key, value} records.
+hipError_t launch_pack_soa(const void *keys, const void *vals, void *out, uint64_t n, hipStream_t s);
+hipError_t launch_unpack_soa(const void *in, void *keys, void *vals, uint64_t n, hipStream_t s);
 
 // Counter-based input generator (gen_kernels.hip): width 4, 8 or 16 (KV);
 // cdf: the Zipf table (dist 1), starts/nb: Pareto burst starts (dist 9).

@@ -56,6 +56,7 @@ def lib():
         L.ogs_sort.argtypes = [vp, i, vp, vp, u64, ctypes.POINTER(vp), vp]
         L.ogs_sort_chunks.argtypes = [vp, i, vp, vp, u64, u64, ctypes.POINTER(vp), vp]
         L.ogs_sort_keep.argtypes = [vp, i, vp, vp, vp, u64, ctypes.POINTER(vp), vp]
+        L.ogs_sort_soa.argtypes = [vp, i, vp, vp, vp, u64, vp]
         L.ogs_profile_enable.argtypes = [vp, i]
         L.ogs_profile_read.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
                                        ctypes.POINTER(u64)]
@@ -230,6 +231,11 @@ class Context:
         check(lib().ogs_sort_keep(self.h, kind, ctypes.c_void_p(src), ctypes.c_void_p(data),
                                   ctypes.c_void_p(scratch), n, ctypes.byref(out), ctypes.c_void_p(stream)))
         return out.value
+
+    def sort_soa_ptr(self, kind, keys, values, work, n, stream=0):
+        """KV sort of separate key and value arrays (ogs_sort_soa); work: 32*n bytes."""
+        check(lib().ogs_sort_soa(self.h, kind, ctypes.c_void_p(keys), ctypes.c_void_p(values), ctypes.c_void_p(work),
+                                 n, ctypes.c_void_p(stream)))
 
     def profile(self, on):
         check(lib().ogs_profile_enable(self.h, 1 if on else 0))

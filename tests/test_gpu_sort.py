@@ -211,6 +211,36 @@ def test_host_entry_point(ctx):
     assert np.array_equal(got, np.sort(a))
 
 
+@pytest.mark.parametrize("kind", KV)
+@pytest.mark.parametrize("n", [0, 1, 2, 1000, 100003, (1 << 20) + 5])
+def test_structure_of_arrays_kv(ctx, kind, n):
+    # ogs_sort_soa: keys and values in separate arrays, packed, sorted and
+    # unpacked on the device; checked like the AoS sort
+    import torch
+    say(f"case soa kind={ogs.KIND_NAMES[kind]} n={n}")
+    a = ogs.generate(kind, ogs.ZIPF if n % 2 else ogs.UNIFORM, n, seed=n + 3)
+    keys = np.ascontiguousarray(a["key"])
+    vals = np.ascontiguousarray(a["value"]) * np.uint64(7) if kind == ogs.KV_U64U64 else \
+        np.ascontiguousarray(a["value"]) * np.int64(-7)
+    a = a.copy()
+    a["value"] = vals
+    dk = torch.from_numpy(keys.view(np.int64).copy()).cuda() if n else torch.zeros(2, dtype=torch.int64, device="cuda")
+    dv = torch.from_numpy(vals.view(np.int64).copy()).cuda() if n else torch.zeros(2, dtype=torch.int64, device="cuda")
+    work = torch.empty(max(32 * n, 16), dtype=torch.uint8, device="cuda")
+    ctx.sort_soa_ptr(kind, dk.data_ptr(), dv.data_ptr(), work.data_ptr(), n)
+    torch.cuda.synchronize()
+    got = np.empty(n, dtype=ogs.NP_DTYPE[kind])
+    got["key"] = dk[:n].cpu().numpy().view(got["key"].dtype)
+    got["value"] = dv[:n].cpu().numpy().view(got["value"].dtype)
+    if n:
+        assert_same(kind, got, a)
+
+
+def test_sort_soa_rejects_key_only_kinds(ctx):
+    with pytest.raises(ogs.OgsError):
+        ctx.sort_soa_ptr(ogs.U64, 16, 16, 16, 4)
+
+
 def test_sort_is_repeatable(ctx):
     # the same input sorted twice gives the same bytes (also for KV ties)
     say("case repeatability")
