@@ -90,12 +90,12 @@ int ogs_sort_keep(ogs_ctx *ctx, ogs_key_t kt, const void *d_in, void *d_data, vo
 
 /* Key-value sort of a structure of arrays: n keys in d_keys and n values in
  * d_values, 8 bytes each (kt: OGS_KV_U64U64 or OGS_KV_I64I64).  d_work is
- * 32 * n bytes of device scratch.  The pairs are packed into the 16-byte
- * records of KeyValue<i64,i64> (Origami/commons.h:51-72) in d_work, sorted
- * there, and the sorted keys and values written back to d_keys and d_values,
- * all ordered on `stream`: two passes more than an AoS sort.  The reference
- * has no such form (its KV type is AoS only); this is the layout adapter
- * for callers that hold keys and values apart. */
+ * 32 * n bytes of device scratch.  The block sort reads the two arrays, the
+ * passes between run on the 16-byte records of KeyValue<i64,i64>
+ * (Origami/commons.h:51-72) in d_work, and the last pass writes the sorted
+ * keys and values back to d_keys and d_values, all ordered on `stream`: the
+ * same pass count as the AoS sort.  The reference has no such form (its KV
+ * type is AoS only); this serves callers that hold keys and values apart. */
 int ogs_sort_soa(ogs_ctx *ctx, ogs_key_t kt, void *d_keys, void *d_values, void *d_work, uint64_t n,
                  void *stream);
 

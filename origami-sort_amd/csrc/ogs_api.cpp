@@ -185,9 +185,13 @@ void Profiler::reset() {
 }
 
 // Plain sort.  Returns the buffer holding the result in *out.
-int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s) {
+int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s,
+              const void *vin, void *kout, void *vout) {
     KeyInfo ki;
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    if ((vin || vout) && ki.width != 16) return fail(OGS_ERR_ARG, "separate value arrays need a key-value type");
+    if (vin && !in) return fail(OGS_ERR_ARG, "separate values need the keys as the input");
+    if (!vout != !kout) return fail(OGS_ERR_ARG, "kout and vout go together");
     if (!out) return fail(OGS_ERR_ARG, "d_sorted is NULL");
     if (n == 0) {
         *out = data;
@@ -228,7 +232,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
         ev[1] = prof.get();
         (void)hipEventRecord(ev[0], s);
     }
-    e = bs.run(bin, src, n, ki.codec, one ? ki.codec : 0, s);
+    e = bs.run(bin, one && vout ? kout : src, n, ki.codec, one ? ki.codec : 0, s, vin, one ? vout : nullptr);
     if (e != hipSuccess) return hip_fail(e, "block sort launch");
     if (prof.on) {
         (void)hipEventRecord(ev[1], s);
@@ -262,11 +266,11 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
                 (void)hipEventRecord(ev[0], s);
             }
             if (wy == 4)
-                e = ms.run_quad(src, dst, n, R, last ? ki.codec : 0, part, tune[ki.kc].v[TP_QUAD_LANES], s,
-                                prof.on ? ev[1] : nullptr);
+                e = ms.run_quad(src, last && vout ? kout : dst, n, R, last ? ki.codec : 0, part,
+                                tune[ki.kc].v[TP_QUAD_LANES], s, prof.on ? ev[1] : nullptr, last ? vout : nullptr);
             else
-                e = ms.run(src, dst, n, R, last ? ki.codec : 0, table ? part : nullptr,
-                           tune[ki.kc].v[TP_PARTITION_LANES], s, prof.on ? ev[1] : nullptr);
+                e = ms.run(src, last && vout ? kout : dst, n, R, last ? ki.codec : 0, table ? part : nullptr,
+                           tune[ki.kc].v[TP_PARTITION_LANES], s, prof.on ? ev[1] : nullptr, last ? vout : nullptr);
             R *= (uint64_t)wy;
             if (e != hipSuccess) return hip_fail(e, "merge pass launch");
             if (prof.on) {
@@ -279,7 +283,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
             std::swap(src, dst);
         }
     }
-    *out = src;
+    *out = vout ? kout : src;
     return OGS_OK;
 }
 
@@ -576,15 +580,11 @@ int ogs_sort_soa(ogs_ctx *ctx, ogs_key_t kt, void *d_keys, void *d_values, void 
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;
+    // the block sort reads the two arrays and the last pass writes them back;
+    // the passes between run on 16-byte records in the work space
     char *wa = (char *)d_work, *wb = wa + 16 * n;
-    e = launch_pack_soa(d_keys, d_values, wa, n, s);
-    if (e != hipSuccess) return hip_fail(e, "pack launch");
     void *res = nullptr;
-    int rc = c->sort(kt, nullptr, wa, wb, n, &res, s);
-    if (rc) return rc;
-    e = launch_unpack_soa(res, d_keys, d_values, n, s);
-    if (e != hipSuccess) return hip_fail(e, "unpack launch");
-    return OGS_OK;
+    return c->sort(kt, d_keys, wa, wb, n, &res, s, d_values, d_keys, d_values);
 }
 
 int ogs_profile_enable(ogs_ctx *ctx, int on) {

@@ -138,7 +138,10 @@ struct Ctx {
     const MergeShapeFns &merge_shape(KeyClass kc, int param, uint64_t block_tile);
     // in == nullptr: sort data in place (data/scratch ping-pong); otherwise read
     // the input from `in` (left intact) and ping-pong between data and scratch.
-    int sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s);
+    // vin: KV values apart from the keys in `in`; kout/vout: the sorted keys and
+    // values written to separate arrays by the last pass (structure of arrays)
+    int sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s,
+             const void *vin = nullptr, void *kout = nullptr, void *vout = nullptr);
     int segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg, void **out,
                   hipStream_t s);
     int batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s);

@@ -470,12 +470,23 @@ __device__ __forceinline__ void lds_put_desc(K *sk, int top, const K (&r)[IPT]) 
 
 // Writes items [0, cnt) of an LDS tile to out[dst ..] with lane-consecutive,
 // fully coalesced stores (16-byte stores when the destination is aligned),
-// applying the output codec.
+// applying the output codec.  KV tiles go to {key, value} records, or, with
+// vout set, keys to out[] and values to vout[] (structure of arrays).
 template <int BLOCK, int TILE, bool HV, typename K>
 __device__ __forceinline__ void lds_store_tile(const K *sk, const uint64_t *sv, void *out, uint64_t dst, int cnt,
-                                               int cout) {
+                                               int cout, void *vout = nullptr) {
     const int tid = threadIdx.x;
     if constexpr (HV) {
+        if (vout) {
+            K *ko = (K *)out + dst;
+            uint64_t *vo = (uint64_t *)vout + dst;
+#pragma unroll 4
+            for (int e = tid; e < cnt; e += BLOCK) {
+                ko[e] = dec_key(sk[lsw<K>(e)], cout);
+                vo[e] = sv[lsw<uint64_t>(e)];
+            }
+            return;
+        }
         ulonglong2 *o = (ulonglong2 *)out + dst;
 #pragma unroll 4
         for (int e = tid; e < cnt; e += BLOCK) {

@@ -23,7 +23,8 @@ namespace ogs {
 // ----------------------------------------------------------------------------
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t base, int cnt, int cin, int cout,
-                                          K *sk, uint64_t *sv, int *smisc, int *sa) {
+                                          K *sk, uint64_t *sv, int *smisc, int *sa, const void *vin = nullptr,
+                                          void *vout = nullptr) {
     constexpr int TILE = BLOCK * IPT;
     constexpr K KMAX = KeyMax<K>::v;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -32,11 +33,21 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     int nmax = 0;
     // all IPT loads are issued before any is used (indices past the end are
     // clamped to the last item, then replaced by pads)
+    if (HV && vin) {
+        // structure of arrays: keys from in[], values from vin[]
 #pragma unroll
-    for (int i = 0; i < IPT; i++) {
-        const int e = i * BLOCK + tid;
-        v[i] = 0;
-        GMem<K, HV>::load(in, base + (e < cnt ? e : cnt - 1), r[i], v[i]);
+        for (int i = 0; i < IPT; i++) {
+            const uint64_t e = base + (uint64_t)(i * BLOCK + tid < cnt ? i * BLOCK + tid : cnt - 1);
+            r[i] = ((const K *)in)[e];
+            v[i] = ((const uint64_t *)vin)[e];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int e = i * BLOCK + tid;
+            v[i] = 0;
+            GMem<K, HV>::load(in, base + (e < cnt ? e : cnt - 1), r[i], v[i]);
+        }
     }
 #pragma unroll
     for (int i = 0; i < IPT; i++) {
@@ -60,7 +71,12 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
                     const int e = i * BLOCK + tid;
                     if (e < cnt && r[i] == KMAX) {
                         int slot = atomicAdd(&smisc[1], 1);
-                        GMem<K, HV>::store(out, base + lim + slot, dec_key(r[i], cout), v[i]);
+                        if (vout) {
+                            ((K *)out)[base + lim + slot] = dec_key(r[i], cout);
+                            ((uint64_t *)vout)[base + lim + slot] = v[i];
+                        } else {
+                            GMem<K, HV>::store(out, base + lim + slot, dec_key(r[i], cout), v[i]);
+                        }
                     }
                 }
             }
@@ -99,11 +115,12 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     __syncthreads();
     lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
     __syncthreads();
-    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, base, lim, cout);
+    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, base, lim, cout, vout);
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-__global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out, uint64_t n, int cin, int cout) {
+__global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out, uint64_t n, int cin, int cout,
+                                                      const void *vin, void *vout) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
@@ -112,7 +129,7 @@ __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out,
     K *sk = skb + lds_front<K>();
     const uint64_t base = (uint64_t)blockIdx.x * TILE;
     const int cnt = (int)(n - base < (uint64_t)TILE ? n - base : (uint64_t)TILE);
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc, sa);
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc, sa, vin, vout);
 }
 
 // Batched tile sort: descriptor i sorts one tile -- a whole small segment, or
@@ -216,7 +233,7 @@ __global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint6
 template <typename K, bool HV, int BLOCK, int IPT>
 __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__restrict__ out, uint64_t seg,
                                            uint64_t n, uint64_t R, uint64_t o0, int cout, K *sk, uint64_t *sv,
-                                           uint64_t *sco, int *sa, const uint64_t *part) {
+                                           uint64_t *sco, int *sa, const uint64_t *part, void *vout = nullptr) {
     static_assert(BLOCK >= 128, "the co-rank searches need two waves");
     constexpr int TILE = BLOCK * IPT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -309,12 +326,13 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
     __syncthreads();
-    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, seg + o0, cnt, cout);
+    lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, seg + o0, cnt, cout, vout);
 }
 
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
-                                                      uint64_t R, int cout, const uint64_t *__restrict__ part) {
+                                                      uint64_t R, int cout, const uint64_t *__restrict__ part,
+                                                      void *vout) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
@@ -322,7 +340,7 @@ __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ i
     __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
     merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco, sa,
-                                  part ? part + blockIdx.x : nullptr);
+                                  part ? part + blockIdx.x : nullptr, vout);
 }
 
 // Batched form: descriptor t = one output tile of one segment that still has
@@ -378,12 +396,13 @@ __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__res
 // the tile shape per size class from the tuning table.
 // ----------------------------------------------------------------------------
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-static hipError_t launch_block(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s) {
+static hipError_t launch_block(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s,
+                              const void *vin, void *vout) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     const uint64_t grid = (n + TILE - 1) / TILE;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL((k_block_sort<K, HV, BLOCK, IPT, WLEV>), dim3((unsigned)grid), dim3(BLOCK), 0, s, in, out, n,
-                       cin, cout);
+                       cin, cout, vin, vout);
     return hipGetLastError();
 }
 
@@ -435,14 +454,14 @@ static void launch_seg_partition(int lanes, const void *tiles, uint64_t ntiles, 
 
 template <typename K, bool HV, int BLOCK, int IPT>
 static hipError_t launch_merge(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part, int lanes,
-                               hipStream_t s, hipEvent_t mid) {
+                               hipStream_t s, hipEvent_t mid, void *vout) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     const uint64_t ntiles = (n + TILE - 1) / TILE;
     if (ntiles == 0) return hipSuccess;
     if (part) launch_partition<K, HV>(lanes, in, n, R, (uint32_t)TILE, ntiles, part, s);
     if (mid) (void)hipEventRecord(mid, s);
     hipLaunchKernelGGL((k_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n, R,
-                       cout, (const uint64_t *)part);
+                       cout, (const uint64_t *)part, vout);
     return hipGetLastError();
 }
 
@@ -712,7 +731,7 @@ __device__ __forceinline__ void quad_level1(const K *sk, const uint64_t *sv, int
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restrict__ in, void *__restrict__ out,
                                                            uint64_t n, uint64_t R, int cout,
-                                                           const uint64_t *__restrict__ tab) {
+                                                           const uint64_t *__restrict__ tab, void *vout) {
     constexpr int C = BLOCK * IPT;  // outputs per round
     constexpr int CAP = 2 * C;      // LDS items: two intervals of at most C
     constexpr uint64_t h = C / 2;
@@ -791,7 +810,7 @@ __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restric
     if (dA < cnt) lds_put<IPT, HV>(sk, sv, dA, r0, v0);
     if (two && dB < cnt) lds_put<IPT, HV>(sk, sv, dB, r1, v1);
     __syncthreads();
-    lds_store_tile<BLOCK, CAP, HV>(sk, sv, out, q.g4 + a0 + b0 + c0 + d0, cnt, cout);
+    lds_store_tile<BLOCK, CAP, HV>(sk, sv, out, q.g4 + a0 + b0 + c0 + d0, cnt, cout, vout);
 }
 
 // Slots of one 4-way pass over n items with runs of R and this tile size.
@@ -805,12 +824,12 @@ __host__ __device__ inline uint64_t quad_pass_slots(uint64_t n, uint64_t R, uint
 // loads).  `mid` is recorded between the two.
 template <typename K, bool HV, int BLOCK, int IPT>
 static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
-                              int quad_lanes, hipStream_t s, hipEvent_t mid) {
+                              int quad_lanes, hipStream_t s, hipEvent_t mid, void *vout) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     // the 4-way tile holds 2 * TILE items (keys, and values for KV) in LDS
     constexpr uint64_t lds = 2 * TILE * (sizeof(K) + (HV ? sizeof(uint64_t) : 0));
     if constexpr (lds > 160 * 1024) {
-        (void)in, (void)out, (void)n, (void)R, (void)cout, (void)tab, (void)quad_lanes, (void)s, (void)mid;
+        (void)in, (void)out, (void)n, (void)R, (void)cout, (void)tab, (void)quad_lanes, (void)s, (void)mid, (void)vout;
         return hipErrorInvalidConfiguration;  // this merge shape has no 4-way form
     } else {
         if (n == 0) return hipSuccess;
@@ -832,7 +851,7 @@ static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R,
                                n, R, TILE / 2, nslots, tab);
         if (mid) (void)hipEventRecord(mid, s);
         hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)nwg), dim3(BLOCK), 0, s, in, out, n, R,
-                           cout, (const uint64_t *)tab);
+                           cout, (const uint64_t *)tab, vout);
         return hipGetLastError();
     }
 }

@@ -53,14 +53,17 @@ hipError_t launch_expand(const SegRecH *segs, uint64_t nseg, TileDescH *td0, Til
                          MergeDescH *md, const uint64_t *poff, uint64_t nlarge, uint64_t LT, uint64_t MT, uint32_t w,
                          hipStream_t s);
 
-typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s);
+// vin / vout (KV only; else nullptr): values read from or written to a separate
+// array, keys from `in` / to `out` (structure of arrays).
+typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s,
+                                  const void *vin, void *vout);
 typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s);
 // part == nullptr: co-ranks found inside the merge kernel; else a table of
 // ceil(n / tile) entries that the partition kernel fills first.
 // `mid` (nullable) is recorded between the partition and the merge kernel.
 // `lanes` = lanes per tile of the partition kernel (1, 4, 8, 16).
 typedef hipError_t (*MergeLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *part,
-                                  int lanes, hipStream_t s, hipEvent_t mid);
+                                  int lanes, hipStream_t s, hipEvent_t mid, void *vout);
 // part: ntiles entries for the descriptors' start co-ranks (filled by a
 // partition kernel first), or nullptr to search inside the merge kernel.
 typedef hipError_t (*SegMergeLaunch)(const void *tiles, uint64_t ntiles, uint64_t R, int cout, uint64_t *part,
@@ -76,7 +79,7 @@ struct BlockShapeFns {
 // tab (4 * quad_table_slots entries) is filled, then the merge runs; `mid` is
 // recorded between the two.
 typedef hipError_t (*QuadLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
-                                 int quad_lanes, hipStream_t s, hipEvent_t mid);
+                                 int quad_lanes, hipStream_t s, hipEvent_t mid, void *vout);
 // Tile slots of one 4-way pass (groups of 4R, 1 + 8R/tile slots each).
 inline uint64_t quad_table_slots(uint64_t n, uint64_t R, uint64_t tile) {
     return (n + 4 * R - 1) / (4 * R) * (1 + 8 * R / tile);
@@ -90,11 +93,6 @@ struct MergeShapeFns {
     bool quad_ok;  // the 4-way tile (2 * tile items) fits in LDS
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
-
-// Structure-of-arrays key-value layout (layout_kernels.hip): keys[i], vals[i]
-// <-> 16-byte {key, value} records.
-hipError_t launch_pack_soa(const void *keys, const void *vals, void *out, uint64_t n, hipStream_t s);
-hipError_t launch_unpack_soa(const void *in, void *keys, void *vals, uint64_t n, hipStream_t s);
 
 // Counter-based input generator (gen_kernels.hip): width 4, 8 or 16 (KV);
 // cdf: the Zipf table (dist 1), starts/nb: Pareto burst starts (dist 9).
