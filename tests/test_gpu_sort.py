@@ -367,6 +367,25 @@ def test_four_way_passes(ctx, kind, dist):
             ctx.tune_set(kind, p, v)
 
 
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.KV_U64U64, ogs.I64])
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+def test_four_way_split_searches(ctx, kind, lanes):
+    # every form of the 4-way split table (1, 2, 4 or 8 lanes per split)
+    # gives the same sort
+    old = {p: ctx.tune_get(kind, p) for p in ("plain_fanin", "quad_lanes")}
+    try:
+        ctx.tune_set(kind, "plain_fanin", 4)
+        ctx.tune_set(kind, "quad_lanes", lanes)
+        for n, dist in ((300007, ogs.ZIPF), ((1 << 21) + 9, ogs.DISTINCT16), (1 << 22, ogs.UNIFORM),
+                        ((1 << 20) + 1, ogs.SORTED), (777777, ogs.REVERSE)):
+            say(f"case 4-way split kind={ogs.KIND_NAMES[kind]} lanes={lanes} n={n} dist={dist}")
+            a = ogs.generate(kind, dist, n, seed=n + lanes)
+            assert_same(kind, gpu_sort(ctx, kind, a), a)
+    finally:
+        for p, v in old.items():
+            ctx.tune_set(kind, p, v)
+
+
 @pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64])
 def test_four_way_every_merge_shape(ctx, kind):
     mshapes = ogs.shapes(kind, 1)
