@@ -149,4 +149,3 @@ def test_quad_split_catches_a_wrong_slot_rule():
 
     rng = random.Random(7)
     assert any(not bad(*make_runs(rng, 8, "uniform", False), 4) for _ in range(50))
-
