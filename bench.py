@@ -317,6 +317,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true",
                     help="after timing, run three 1-input-sized device copies (torch copy_) as the "
                          "known-size reference kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE")
+    ap.add_argument("--tune", action="append", default=[], metavar="PARAM=VALUE",
+                    help="override one tile-table entry of the benchmarked key class (repeatable)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -338,6 +340,9 @@ def main():
     dname = args.dist or ("normal" if kind == ogs.F32 else "uniform")
     dist = dists[dname]
     ctx = ogs.Context(0)
+    for t in args.tune:
+        pname, pval = t.split("=")
+        ctx.tune_set(kind, pname, int(pval))
     sample_n = min(n, 1 << 26)
     if dist < ogs.SAME:
         log(f"bench: generating {n} {args.kind} keys, {dname} (seed 42)")
@@ -390,7 +395,8 @@ def main():
     value = n / (ms_step / 1e3) / 1e9
 
     kernels = {}
-    for kid, name in ((0, "block_sort"), (1, "merge_partition"), (2, "merge_pass"), (3, "merge4_pass")):
+    for kid, name in ((0, "block_sort"), (1, "merge_partition"), (2, "merge_pass"), (3, "merge4_pass"),
+                      (4, "kway_pass")):
         ms, cnt, items = ctx.profile_read(kid)
         if cnt:
             kernels[name] = {"total_ms": ms, "launches": cnt, "mean_ms": ms / cnt, "items": items}

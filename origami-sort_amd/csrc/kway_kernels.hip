@@ -1,0 +1,313 @@
+// kway_kernels.hip -- the sample-split K-way merge pass (key-only classes).
+//
+// One HBM pass merges groups of KW sorted runs (KW = 2, 4, 8, 16): log2(KW)
+// levels of the reference's merge tree (MergeTreeEven / Merge4Way,
+// merge_tree.h:319-3010) per read and write of the data, with the tiles cut by
+// the role of Partition_Ptrs (Partition.h:117-166) for KW runs at once.
+//
+// Tiles.  The pass before this one wrote every s-th item of each run (s =
+// 2^slog2) as a tagged sample (key, run, index); the tagged samples of each
+// group are merged (a few 2-way passes over n/s samples, run by the host), so
+// the group's samples are in the total order (key, run, index).  The tile
+// boundaries of a group are its merged samples number 0, Q, 2Q, ...: for a
+// boundary item b = (v, r, m*s) every run j is split exactly at the number of
+// its items that precede b in the total order -- the items <= v for j < r, the
+// items < v for j > r, and m*s for j = r.  Those counts are found by a binary
+// search over run j's own samples, then one over the s-item window of run j
+// between two samples.  Between two boundaries Q samples apart each run
+// contributes at most (its samples there + 1) * s - 1 items, so a tile holds at
+// most (Q + KW) * s - KW items and at least (Q - KW) * s; Q is chosen so the
+// tile, with each of its KW slices padded to a multiple of IPT, fits the
+// BLOCK * IPT items of LDS.
+//
+// Merge.  The workgroup loads the KW slices into LDS, each padded with maximum
+// keys to a multiple of IPT (so every pair of runs at every level starts on a
+// thread boundary and no thread's IPT outputs straddle two pairs), the B side
+// of every pair stored descending (the bitonic layout of the block sort's
+// rounds, so merges test no exhaustion), merges the pairs level by level
+// through registers, and writes the first cnt items (the real ones: pads are
+// maximum keys and sort last; for key-only data equal keys are identical bits)
+// to the output at the group base plus the sum of the tile's start splits.
+#include "sort_core.hpp"
+#include "sort_kernels.hpp"
+
+namespace ogs {
+
+// Items of run j of the group starting at gbase (KW runs of R, ragged at n).
+__host__ __device__ inline uint64_t kw_run_len(uint64_t n, uint64_t gbase, uint64_t R, int j) {
+    const uint64_t st = gbase + (uint64_t)j * R;
+    return n > st ? (n - st < R ? n - st : R) : 0;
+}
+
+// Merged samples between two tile boundaries (0 if the shape cannot hold KW
+// padded slices at this spacing).
+__host__ __device__ inline uint64_t kw_q(uint64_t tile, int ipt, int kw, int slog2) {
+    const int64_t q = ((int64_t)tile - (int64_t)kw * ipt) / ((int64_t)1 << slog2) - kw;
+    return q > 0 ? (uint64_t)q : 0;
+}
+
+// Tiles of a group with `ns` merged samples.
+__host__ __device__ inline uint64_t kw_group_tiles(uint64_t ns, uint64_t q) { return (ns + q - 1) / q; }
+
+// Merged samples of the last group of n items (KW runs of R).
+__host__ __device__ inline uint64_t kw_last_samples(uint64_t n, int rlog2, int slog2, int kw) {
+    const uint64_t R = 1ull << rlog2, GR = R * (uint64_t)kw;
+    const uint64_t gbase = (n - 1) / GR * GR;
+    uint64_t ns = 0;
+    for (int j = 0; j < kw; j++) ns += (kw_run_len(n, gbase, R, j) + (1ull << slog2) - 1) >> slog2;
+    return ns;
+}
+
+uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int ipt) {
+    if (n == 0) return 0;
+    const uint64_t q = kw_q(tile, ipt, kw, slog2);
+    if (q == 0) return 0;
+    const uint64_t GR = (1ull << rlog2) * (uint64_t)kw;
+    const uint64_t groups = (n + GR - 1) / GR;
+    const uint64_t ntg = kw_group_tiles(GR >> slog2, q);
+    return (groups - 1) * ntg + kw_group_tiles(kw_last_samples(n, rlog2, slog2, kw), q);
+}
+
+// ----------------------------------------------------------------------------
+// Tile splits: KW lanes per tile (lane j: run j); tab[t * KW + j] = the split
+// of run j at tile t's first item.  Each lane runs two binary searches: over
+// run j's samples (which number precede the boundary), then over the s items
+// that separate two of them.
+// ----------------------------------------------------------------------------
+template <typename K, int KW>
+__global__ void k_kway_bounds(const K *__restrict__ in, uint64_t n, int rlog2, int slog2, uint64_t Q, uint64_t ntg,
+                              uint64_t ntiles, const void *__restrict__ s0, const void *__restrict__ sm,
+                              uint64_t *__restrict__ tab) {
+    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t t = gt / KW;
+    const int j = (int)(gt % KW);
+    if (t >= ntiles) return;
+    const uint64_t R = 1ull << rlog2;
+    const uint64_t g = t / ntg, tl = t % ntg;
+    const uint64_t gbase = g * KW * R;
+    const uint64_t lenj = kw_run_len(n, gbase, R, j);
+    uint64_t split = 0;
+    if (tl > 0 && lenj > 0) {
+        K v;
+        int r;
+        uint64_t m;
+        Sample<K>::get(sm, (gbase >> slog2) + tl * Q, v, r, m);
+        if (j == r) {
+            split = m << slog2;
+        } else {
+            // run j's items precede b: those <= v when j < r, those < v when j > r
+            const bool up = j < r;
+            const uint64_t sb = (gbase + (uint64_t)j * R) >> slog2;
+            uint64_t lo = 0, hi = (lenj + (1ull << slog2) - 1) >> slog2;
+            while (lo < hi) {
+                const uint64_t mid = (lo + hi) >> 1;
+                const K k = Sample<K>::key(s0, sb + mid);
+                if (up ? !(v < k) : k < v) lo = mid + 1;
+                else hi = mid;
+            }
+            if (lo > 0) {
+                // sample lo-1 precedes b and sample lo (if any) does not
+                const K *X = in + gbase + (uint64_t)j * R;
+                uint64_t a = ((lo - 1) << slog2) + 1;
+                uint64_t b = lo << slog2;
+                b = b < lenj ? b : lenj;
+                while (a < b) {
+                    const uint64_t mid = (a + b) >> 1;
+                    const K k = X[mid];
+                    if (up ? !(v < k) : k < v) a = mid + 1;
+                    else b = mid;
+                }
+                split = a;
+            }
+        }
+    }
+    tab[t * KW + j] = split;
+}
+
+// ----------------------------------------------------------------------------
+// The merge: one workgroup per tile.
+// ----------------------------------------------------------------------------
+template <typename K, int KW, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, K *__restrict__ out, uint64_t n,
+                                                      int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
+                                                      const uint64_t *__restrict__ tab, SampleSpec sp) {
+    static_assert(KW == 2 || KW == 4 || KW == 8 || KW == 16, "fan-in 2, 4, 8 or 16");
+    static_assert((IPT & (IPT - 1)) == 0 && IPT >= 4, "IPT: a power of two >= 4");
+    constexpr int C = BLOCK * IPT;
+    constexpr int LOGKW = KW == 2 ? 1 : KW == 4 ? 2 : KW == 8 ? 3 : 4;
+    constexpr K KMAX = KeyMax<K>::v;
+    __shared__ __attribute__((aligned(16))) K skb[C + lds_extra<K, IPT>()];
+    __shared__ uint64_t s_src[KW];  // global index of each slice's first item
+    __shared__ int s_n[KW];         // items per slice
+    __shared__ int s_P[KW + 1];     // prefix sums of the padded slice lengths
+    __shared__ uint64_t s_cb[BLOCK];  // per chunk of IPT LDS items: source of its item 0
+    __shared__ int s_cm[BLOCK];       // per chunk: first and end real position, direction
+    K *sk = skb + lds_front<K>();
+    const int tid = threadIdx.x;
+    const uint64_t t = blockIdx.x;
+    const uint64_t R = 1ull << rlog2;
+    const uint64_t g = t / ntg, tl = t % ntg;
+    const uint64_t gbase = g * KW * R;
+    const uint64_t nt_here = gbase + KW * R >= n ? nt_last : ntg;
+    if (tid < KW) {
+        const uint64_t lenj = kw_run_len(n, gbase, R, tid);
+        const uint64_t lo = tab[t * KW + tid];
+        const uint64_t hi = tl + 1 < nt_here ? tab[(t + 1) * KW + tid] : lenj;
+        s_src[tid] = gbase + (uint64_t)tid * R + lo;
+        // a split table that is not monotone gives a negative count: the tile is skipped
+        s_n[tid] = hi >= lo && hi <= lenj ? (int)(hi - lo) : -1;
+    }
+    __syncthreads();
+    int P[KW + 1];
+    int cnt = 0;
+    uint64_t o0 = gbase;
+    bool bad = false;
+    P[0] = 0;
+#pragma unroll
+    for (int j = 0; j < KW; j++) {
+        const int nj = s_n[j];
+        bad |= nj < 0;
+        cnt += nj;
+        P[j + 1] = P[j] + ((nj + IPT - 1) & ~(IPT - 1));
+        o0 += s_src[j] - (gbase + (uint64_t)j * R);
+    }
+    const int Np = P[KW];
+    if (bad || Np > C) return;  // uniform: cannot happen with a consistent table
+    if (tid == 0) {
+#pragma unroll
+        for (int j = 0; j <= KW; j++) s_P[j] = P[j];
+    }
+    // this thread's chunk [c0, c0 + IPT) lies in slice j0
+    const int c0 = tid * IPT;
+    const bool live = c0 < Np;
+    int j0 = 0;
+#pragma unroll
+    for (int j = 1; j < KW; j++) j0 += P[j] <= c0 ? 1 : 0;
+    if (live) {
+        // odd slices are the B side of the first level: stored descending
+        int pj = 0, npj = 0, nj = 0;
+        uint64_t src = 0;
+#pragma unroll
+        for (int j = 0; j < KW; j++)
+            if (j == j0) {
+                pj = P[j];
+                npj = P[j + 1] - P[j];
+                nj = s_n[j];
+                src = s_src[j];
+            }
+        const int q = c0 - pj;
+        const bool desc = (j0 & 1) != 0;
+        int v0, v1;
+        uint64_t base;
+        if (desc) {
+            // position q + r holds item npj - 1 - q - r, real when < nj
+            base = src + (uint64_t)(npj - 1 - q);
+            v0 = npj - q - nj;
+            v0 = v0 < 0 ? 0 : v0 > IPT ? IPT : v0;
+            v1 = IPT;
+        } else {
+            base = src + (uint64_t)q;
+            v0 = 0;
+            v1 = nj - q;
+            v1 = v1 < 0 ? 0 : v1 > IPT ? IPT : v1;
+        }
+        s_cb[tid] = base;
+        s_cm[tid] = v0 | (v1 << 8) | ((int)desc << 16);
+    }
+    __syncthreads();
+    {
+        // all loads in flight at once; pads and positions past the tile read
+        // the group's first item and keep the maximum key
+        K kk[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int e = i * BLOCK + tid;
+            const int ec = e < Np ? e : 0;
+            const int c = ec / IPT, r = ec % IPT;
+            const uint64_t b = s_cb[c];
+            const int m = s_cm[c];
+            const bool real = e < Np && r >= (m & 255) && r < ((m >> 8) & 255);
+            const uint64_t src = real ? ((m >> 16) ? b - (uint64_t)r : b + (uint64_t)r) : gbase;
+            const K x = in[src];
+            kk[i] = real ? x : KMAX;
+        }
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const int e = i * BLOCK + tid;
+            if (e < Np) sk[lsw<K>(e)] = kk[i];
+        }
+    }
+    __syncthreads();
+    K r[IPT];
+    uint64_t vdum[IPT];
+#pragma unroll 1
+    for (int lev = 1; lev <= LOGKW; lev++) {
+        int end = 0, diag = 0;
+        if (live) {
+            const int hv = 1 << (lev - 1);
+            const int s0 = (j0 >> lev) << lev;
+            const int a_off = s_P[s0], mid = s_P[s0 + hv];
+            end = s_P[s0 + 2 * hv];
+            const int la = mid - a_off, lb = end - mid;
+            diag = c0 - a_off;
+            lds_merge_slice_rev<IPT>(sk, a_off, la, end - 1, lb, diag, IPT, pow2_floor(la < lb ? la : lb), r);
+        }
+        __syncthreads();
+        if (live) {
+            // the next level's B sides are stored descending
+            if (lev < LOGKW && ((j0 >> lev) & 1)) lds_put_desc<IPT>(sk, end - 1 - diag, r);
+            else lds_put<IPT, false>(sk, nullptr, c0, r, vdum);
+        }
+        __syncthreads();
+    }
+#pragma unroll 4
+    for (int e = tid; e < cnt; e += BLOCK) out[o0 + e] = dec_key(sk[lsw<K>(e)], cout);
+    if (sp.smp) write_samples<BLOCK>(sk, cnt, o0, sp.smp, sp.rlog2, sp.kw, sp.slog2);
+}
+
+template <typename K, int KW, int BLOCK, int IPT>
+static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, int slog2, int cout, uint64_t *tab,
+                              const void *s0, const void *sm, const SampleSpec *next, hipStream_t s, hipEvent_t mid) {
+    constexpr uint64_t C = (uint64_t)BLOCK * IPT;
+    if (n == 0) return hipSuccess;
+    const uint64_t Q = kw_q(C, IPT, KW, slog2);
+    if (Q == 0 || rlog2 - slog2 > 28 || rlog2 < slog2) return hipErrorInvalidValue;
+    const uint64_t GR = (1ull << rlog2) * KW;
+    const uint64_t ntg = kw_group_tiles(GR >> slog2, Q);
+    const uint64_t nt_last = kw_group_tiles(kw_last_samples(n, rlog2, slog2, KW), Q);
+    const uint64_t ntiles = (n + GR - 1) / GR * ntg - ntg + nt_last;
+    const uint64_t threads = ntiles * KW;
+    hipLaunchKernelGGL((k_kway_bounds<K, KW>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, (const K *)in,
+                       n, rlog2, slog2, Q, ntg, ntiles, s0, sm, tab);
+    if (mid) (void)hipEventRecord(mid, s);
+    const SampleSpec none = {nullptr, 0, 0, 0};
+    hipLaunchKernelGGL((k_kway_merge<K, KW, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, (const K *)in,
+                       (K *)out, n, rlog2, ntg, nt_last, cout, (const uint64_t *)tab, next ? *next : none);
+    return hipGetLastError();
+}
+
+#define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16)
+#define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8)
+#define OGS_KX(K, B, I) \
+    {B, I, {&launch_kway<K, 2, B, I>, &launch_kway<K, 4, B, I>, &launch_kway<K, 8, B, I>, &launch_kway<K, 16, B, I>}},
+
+static const KwayShapeFns kKway32[] = {
+#define X(B, I) OGS_KX(uint32_t, B, I)
+    OGS_KWAY_SHAPES_32(X)
+#undef X
+};
+static const KwayShapeFns kKway64[] = {
+#define X(B, I) OGS_KX(uint64_t, B, I)
+    OGS_KWAY_SHAPES_64(X)
+#undef X
+};
+
+const KwayShapeFns *kway_shapes(KeyClass kc, int *count) {
+    switch (kc) {
+    case KC_32: *count = (int)(sizeof(kKway32) / sizeof(kKway32[0])); return kKway32;
+    case KC_64: *count = (int)(sizeof(kKway64) / sizeof(kKway64[0])); return kKway64;
+    default: *count = 0; return nullptr;
+    }
+}
+
+}  // namespace ogs

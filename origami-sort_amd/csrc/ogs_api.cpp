@@ -70,6 +70,9 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_PARTITION_LANES] = 4;
     t.v[TP_PLAIN_FANIN] = 2;
     t.v[TP_QUAD_LANES] = 1;
+    t.v[TP_KWAY_FANIN] = 0;
+    t.v[TP_KWAY_SHAPE] = 0;
+    t.v[TP_KWAY_SAMPLE] = 5;
     return t;
 }
 
@@ -203,6 +206,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     // before writing it back
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if (!vin && !vout && kway_on(ki, n)) return sort_kway(ki, in, data, scratch, n, out, s);
     const BlockShapeFns &bs = block_shape(ki.kc, TP_PLAIN_BLOCK);
     const MergeShapeFns &ms = merge_shape(ki.kc, TP_PLAIN_MERGE, bs.tile());
     const uint64_t T = bs.tile();
@@ -232,7 +236,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
         ev[1] = prof.get();
         (void)hipEventRecord(ev[0], s);
     }
-    e = bs.run(bin, one && vout ? kout : src, n, ki.codec, one ? ki.codec : 0, s, vin, one ? vout : nullptr);
+    e = bs.run(bin, one && vout ? kout : src, n, ki.codec, one ? ki.codec : 0, s, vin, one ? vout : nullptr, nullptr);
     if (e != hipSuccess) return hip_fail(e, "block sort launch");
     if (prof.on) {
         (void)hipEventRecord(ev[1], s);
@@ -287,10 +291,143 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     return OGS_OK;
 }
 
+static int ilog2(uint64_t x) { return 63 - __builtin_clzll(x); }
+
+bool Ctx::kway_on(const KeyInfo &ki, uint64_t n) {
+    if (ki.kc == KC_KV || tune[ki.kc].v[TP_KWAY_FANIN] < 2) return false;
+    return n > block_shape(ki.kc, TP_PLAIN_BLOCK).tile();
+}
+
+// Merge levels per K-way pass: ceil(levels / log2(fan-in)) passes, the levels
+// spread evenly over them, the larger passes first.
+std::vector<int> Ctx::kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile) {
+    int L = 0;
+    for (uint64_t R = block_tile; R < n; R <<= 1) L++;
+    const int lmax = ilog2((uint64_t)tune[kc].v[TP_KWAY_FANIN]);
+    const int P = (L + lmax - 1) / lmax;
+    std::vector<int> lv(P);
+    for (int i = 0; i < P; i++) lv[i] = L / P + (i < L % P ? 1 : 0);
+    return lv;
+}
+
+// Plain key-only sort with sample-split K-way passes.  Pass p merges groups of
+// 2^lv[p] runs; its tile splits come from every 2^sl[p]-th item of each input
+// run (written as tagged samples by the kernel that wrote the run), merged per
+// group by 2-way merge passes over the sample array.  Sample buffers: S0 holds
+// the unmerged samples of the current input; S1 and S2 take the sample merges.
+int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch, uint64_t n, void **out,
+                   hipStream_t s) {
+    const BlockShapeFns &bs = block_shape(ki.kc, TP_PLAIN_BLOCK);
+    int kcnt = 0;
+    const KwayShapeFns *kt = kway_shapes(ki.kc, &kcnt);
+    int kidx = tune[ki.kc].v[TP_KWAY_SHAPE];
+    if (kidx < 0 || kidx >= kcnt) kidx = 0;
+    const KwayShapeFns &ks = kt[kidx];
+    const uint64_t T = bs.tile();
+    if (T & (T - 1)) return fail(OGS_ERR_ARG, "K-way passes need a power-of-two block tile");
+    const std::vector<int> lv = kway_plan(ki.kc, n, T);
+    const int P = (int)lv.size();
+    // sample spacing per pass: at most the tuned one, and at most R / 256 so
+    // the smallest 2-way sample merge tile (512) divides every pair span
+    std::vector<int> sl(P);
+    int rl = ilog2(T), smin = 63;
+    for (int p = 0; p < P; p++) {
+        sl[p] = std::min(tune[ki.kc].v[TP_KWAY_SAMPLE], rl - 8);
+        if (sl[p] < 0 || rl - sl[p] > 28)
+            return fail(OGS_ERR_ARG, "K-way pass %d: no sample spacing for runs of 2^%d", p, rl);
+        smin = std::min(smin, sl[p]);
+        rl += lv[p];
+    }
+    // sample records: u64 (32-bit keys) merged as u64 keys, {key, tag} (64-bit
+    // keys) merged as stable key-value records
+    const KeyClass skc = ki.kc == KC_32 ? KC_64 : KC_KV;
+    const uint64_t sw = ki.kc == KC_32 ? 8 : 16;
+    const uint64_t ns_max = (n + (1ull << smin) - 1) >> smin;
+    uint64_t tab_n = 0;
+    rl = ilog2(T);
+    for (int p = 0; p < P; p++) {
+        const uint64_t nt = kway_tiles(n, rl, sl[p], 1 << lv[p], ks.tile(), ks.ipt);
+        if (nt == 0) return fail(OGS_ERR_ARG, "K-way pass %d: the tile shape cannot hold %d slices", p, 1 << lv[p]);
+        tab_n = std::max(tab_n, nt * (uint64_t)(1 << lv[p]));
+        rl += lv[p];
+    }
+    const uint64_t part_n = (ns_max + 511) / 512 + 4;
+    auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
+    const uint64_t o_s1 = al(ns_max * sw), o_s2 = o_s1 + al(ns_max * sw), o_tab = o_s2 + al(ns_max * sw),
+                   o_part = o_tab + al(tab_n * 8), bytes = o_part + al(part_n * 8);
+    Workspace *w = workspace(s);
+    std::lock_guard<std::mutex> g(w->mu);
+    int rc = w->reserve(bytes);
+    if (rc) return rc;
+    char *D = (char *)w->dev;
+    void *S0 = D, *S1 = D + o_s1, *S2 = D + o_s2;
+    uint64_t *tab = (uint64_t *)(D + o_tab), *mpart = (uint64_t *)(D + o_part);
+    void *src = scratch, *dst = data;
+    const void *bin = data;
+    if (in) {
+        bin = in;
+        src = (P & 1) ? scratch : data;
+        dst = (src == data) ? scratch : data;
+    }
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    if (prof.on) {
+        ev[0] = prof.get();
+        ev[1] = prof.get();
+        (void)hipEventRecord(ev[0], s);
+    }
+    rl = ilog2(T);
+    const SampleSpec first = {S0, rl, 1 << lv[0], sl[0]};
+    hipError_t e = bs.run(bin, src, n, ki.codec, 0, s, nullptr, nullptr, &first);
+    if (e != hipSuccess) return hip_fail(e, "block sort launch");
+    if (prof.on) {
+        (void)hipEventRecord(ev[1], s);
+        prof.pending.push_back({PK_BLOCK, ev[0], ev[1], n, true});
+    }
+    int mcnt = 0;
+    const MergeShapeFns *mt = merge_shapes(skc, &mcnt);
+    const int lanes = tune[skc].v[TP_PARTITION_LANES];
+    for (int p = 0; p < P; p++) {
+        const bool last = p == P - 1;
+        if (prof.on) {
+            for (int k = 0; k < 3; k++) ev[k] = prof.get();
+            (void)hipEventRecord(ev[0], s);
+        }
+        // merge the samples of each group of 2^lv[p] runs
+        const uint64_t ns = (n + (1ull << sl[p]) - 1) >> sl[p];
+        const void *sa = S0;
+        void *sb[2] = {S1, S2};
+        for (int q = 0; q < lv[p]; q++) {
+            const uint64_t Rs = 1ull << (rl - sl[p] + q);
+            int best = -1;
+            for (int j = 0; j < mcnt; j++)
+                if (mt[j].tile() <= 2 * Rs && (best < 0 || mt[j].tile() > mt[best].tile())) best = j;
+            if (best < 0) return fail(OGS_ERR_ARG, "no sample merge tile divides 2^%d", (int)ilog2(2 * Rs));
+            e = mt[best].run(sa, sb[q & 1], ns, Rs, 0, mpart, lanes, s, nullptr, nullptr);
+            if (e != hipSuccess) return hip_fail(e, "sample merge launch");
+            sa = sb[q & 1];
+        }
+        SampleSpec next = {nullptr, 0, 0, 0};
+        if (!last) next = {S0, rl + lv[p], 1 << lv[p + 1], sl[p + 1]};
+        e = ks.run[lv[p] - 1](src, dst, n, rl, sl[p], last ? ki.codec : 0, tab, S0, sa, last ? nullptr : &next, s,
+                              prof.on ? ev[1] : nullptr);
+        if (e != hipSuccess) return hip_fail(e, "K-way pass launch");
+        if (prof.on) {
+            (void)hipEventRecord(ev[2], s);
+            prof.pending.push_back({PK_PARTITION, ev[0], ev[1], n, true});
+            prof.pending.push_back({PK_KWAY, ev[1], ev[2], n, false});
+        }
+        rl += lv[p];
+        std::swap(src, dst);
+    }
+    *out = src;
+    return OGS_OK;
+}
+
 int Ctx::pass_count(ogs_key_t kt, uint64_t n) {
     KeyInfo ki;
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
     if (n == 0) return 0;
+    if (kway_on(ki, n)) return 1 + (int)kway_plan(ki.kc, n, block_shape(ki.kc, TP_PLAIN_BLOCK).tile()).size();
     const uint64_t T = block_shape(ki.kc, TP_PLAIN_BLOCK).tile();
     const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4 && merge_shape(ki.kc, TP_PLAIN_MERGE, T).quad_ok;
     int p = 1;

@@ -29,7 +29,8 @@ uint64_t mix64(uint64_t z) {  // splitmix64 finaliser
 const char *kParamNames[TP_COUNT] = {"plain_block",     "plain_merge", "seg_small_block",
                                      "seg_small_max",   "seg_large_block", "seg_merge",
                                      "seg_tiny_block",  "seg_tiny_max", "plain_corank", "seg_plain_min",
-                                     "partition_lanes", "plain_fanin", "quad_lanes"};
+                                     "partition_lanes", "plain_fanin", "quad_lanes", "kway_fanin",
+                                     "kway_shape", "kway_sample"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -224,6 +225,18 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
         reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
         return OGS_OK;
     }
+    else if (p == TP_KWAY_FANIN) {
+        if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
+            return fail(OGS_ERR_ARG, "kway_fanin must be 0, 2, 4, 8 or 16 (got %d)", value);
+        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
+        return OGS_OK;
+    }
+    else if (p == TP_KWAY_SAMPLE) {
+        if (value < 3 || value > 10) return fail(OGS_ERR_ARG, "kway_sample must be in [3, 10] (got %d)", value);
+        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
+        return OGS_OK;
+    }
+    else if (p == TP_KWAY_SHAPE) kway_shapes(ki.kc, &cnt);
     else if (p == TP_PARTITION_LANES) {
         if (value != 1 && value != 4 && value != 8 && value != 16)
             return fail(OGS_ERR_ARG, "partition_lanes must be 1, 4, 8 or 16 (got %d)", value);

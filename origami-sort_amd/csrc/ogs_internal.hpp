@@ -41,7 +41,10 @@ enum TuneParam {
     TP_PARTITION_LANES = 10,// lanes per merge tile in the partition kernel: 1, 4, 8 or 16
     TP_PLAIN_FANIN = 11,    // plain merges: 2 (one level per HBM pass) or 4 (two levels)
     TP_QUAD_LANES = 12,     // lanes per tile slot in the 4-way split kernel: 1, 2, 4 or 8
-    TP_COUNT = 13
+    TP_KWAY_FANIN = 13,     // key-only plain merges: largest fan-in of the sample-split K-way pass (0 = off)
+    TP_KWAY_SHAPE = 14,     // tile shape of the K-way merge kernel
+    TP_KWAY_SAMPLE = 15,    // log2 of the sample spacing of the K-way pass
+    TP_COUNT = 16
 };
 struct Tuning {
     int v[TP_COUNT];
@@ -75,9 +78,9 @@ struct SegSpec {
 
 // Per-kernel timing with HIP events recorded on the launching stream
 // (ogs_profile_*): kernel kinds 0 block sort, 1 merge partition (both the
-// 2-way table and the 4-way sample + split tables), 2 2-way merge pass,
-// 3 4-way merge pass.
-enum ProfKind { PK_BLOCK = 0, PK_PARTITION = 1, PK_MERGE = 2, PK_MERGE4 = 3, PK_COUNT = 4 };
+// 2-way table, the 4-way split table, and the K-way sample merges + split
+// table), 2 2-way merge pass, 3 4-way merge pass, 4 K-way merge pass.
+enum ProfKind { PK_BLOCK = 0, PK_PARTITION = 1, PK_MERGE = 2, PK_MERGE4 = 3, PK_KWAY = 4, PK_COUNT = 5 };
 struct Profiler {
     bool on = false;
     struct Rec {
@@ -88,9 +91,9 @@ struct Profiler {
     };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> pool;
-    double total_ms[PK_COUNT] = {0, 0, 0, 0};
-    uint64_t count[PK_COUNT] = {0, 0, 0, 0};
-    uint64_t items[PK_COUNT] = {0, 0, 0, 0};
+    double total_ms[PK_COUNT] = {};
+    uint64_t count[PK_COUNT] = {};
+    uint64_t items[PK_COUNT] = {};
     hipEvent_t get();
     int flush();  // waits for pending events and accumulates
     void reset();
@@ -142,6 +145,11 @@ struct Ctx {
     // values written to separate arrays by the last pass (structure of arrays)
     int sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s,
              const void *vin = nullptr, void *kout = nullptr, void *vout = nullptr);
+    // key-only plain sort with sample-split K-way merge passes (n > block tile)
+    int sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s);
+    // merge passes of a K-way plain sort: levels per pass (largest first)
+    std::vector<int> kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile);
+    bool kway_on(const KeyInfo &ki, uint64_t n);
     int segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg, void **out,
                   hipStream_t s);
     int batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s);

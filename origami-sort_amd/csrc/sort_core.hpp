@@ -525,6 +525,57 @@ __device__ __forceinline__ void lds_store_tile(const K *sk, const uint64_t *sv, 
 }
 
 // ----------------------------------------------------------------------------
+// Tagged samples for the sample-split K-way pass.  Items [0, cnt) of an LDS
+// tile are output items g0 .. g0+cnt-1; every item whose output index is a
+// multiple of 2^slog2 is written to smp[index >> slog2] with its run r within
+// the next pass's group of kw runs of 2^rlog2 items and its index m in that
+// run over 2^slog2: 32-bit keys as u64 key << 32 | r << 28 | m (u64 order is
+// then the total order (key, run, index)), 64-bit keys as {key, r << 32 | m}.
+// ----------------------------------------------------------------------------
+template <int BLOCK, typename K>
+__device__ __forceinline__ void write_samples(const K *sk, int cnt, uint64_t g0, void *smp, int rlog2, int kw,
+                                              int slog2) {
+    const int s = 1 << slog2;
+    const int first = (int)((0 - g0) & (uint64_t)(s - 1));
+    for (int e = first + (int)threadIdx.x * s; e < cnt; e += BLOCK * s) {
+        const uint64_t gi = g0 + (uint64_t)e;
+        const uint64_t run = gi >> rlog2;
+        const uint64_t r = run & (uint64_t)(kw - 1);
+        const uint64_t m = (gi - (run << rlog2)) >> slog2;
+        const K key = sk[lsw<K>(e)];
+        if constexpr (sizeof(K) == 4)
+            ((uint64_t *)smp)[gi >> slog2] = ((uint64_t)key << 32) | (r << 28) | m;
+        else
+            ((ulonglong2 *)smp)[gi >> slog2] = make_ulonglong2((unsigned long long)key, (r << 32) | m);
+    }
+}
+
+// Reading samples back: the key, and the key with its run and index.
+template <typename K> struct Sample;
+template <> struct Sample<uint32_t> {
+    static __device__ __forceinline__ uint32_t key(const void *p, uint64_t i) {
+        return (uint32_t)(((const uint64_t *)p)[i] >> 32);
+    }
+    static __device__ __forceinline__ void get(const void *p, uint64_t i, uint32_t &k, int &r, uint64_t &m) {
+        const uint64_t x = ((const uint64_t *)p)[i];
+        k = (uint32_t)(x >> 32);
+        r = (int)((x >> 28) & 15);
+        m = x & ((1ull << 28) - 1);
+    }
+};
+template <> struct Sample<uint64_t> {
+    static __device__ __forceinline__ uint64_t key(const void *p, uint64_t i) {
+        return (uint64_t)((const ulonglong2 *)p)[i].x;
+    }
+    static __device__ __forceinline__ void get(const void *p, uint64_t i, uint64_t &k, int &r, uint64_t &m) {
+        const ulonglong2 x = ((const ulonglong2 *)p)[i];
+        k = (uint64_t)x.x;
+        r = (int)(x.y >> 32);
+        m = x.y & 0xFFFFFFFFull;
+    }
+};
+
+// ----------------------------------------------------------------------------
 // Wave-cooperative co-rank in global memory (merge path, Partition.h:117-166
 // specialised to two runs): returns the number of A items among the first
 // `diag` outputs of merge(A, B), ties to A.  Each step evaluates 64 evenly

@@ -24,7 +24,7 @@ namespace ogs {
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t base, int cnt, int cin, int cout,
                                           K *sk, uint64_t *sv, int *smisc, int *sa, const void *vin = nullptr,
-                                          void *vout = nullptr) {
+                                          void *vout = nullptr, const SampleSpec *sp = nullptr) {
     constexpr int TILE = BLOCK * IPT;
     constexpr K KMAX = KeyMax<K>::v;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -116,11 +116,13 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
     __syncthreads();
     lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, base, lim, cout, vout);
+    if constexpr (!HV)
+        if (sp && sp->smp) write_samples<BLOCK>(sk, lim, base, sp->smp, sp->rlog2, sp->kw, sp->slog2);
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out, uint64_t n, int cin, int cout,
-                                                      const void *vin, void *vout) {
+                                                      const void *vin, void *vout, SampleSpec sp) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
@@ -129,7 +131,7 @@ __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out,
     K *sk = skb + lds_front<K>();
     const uint64_t base = (uint64_t)blockIdx.x * TILE;
     const int cnt = (int)(n - base < (uint64_t)TILE ? n - base : (uint64_t)TILE);
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc, sa, vin, vout);
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc, sa, vin, vout, &sp);
 }
 
 // Batched tile sort: descriptor i sorts one tile -- a whole small segment, or
@@ -397,12 +399,13 @@ __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__res
 // ----------------------------------------------------------------------------
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 static hipError_t launch_block(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s,
-                              const void *vin, void *vout) {
+                              const void *vin, void *vout, const SampleSpec *sp) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     const uint64_t grid = (n + TILE - 1) / TILE;
     if (grid == 0) return hipSuccess;
+    const SampleSpec none = {nullptr, 0, 0, 0};
     hipLaunchKernelGGL((k_block_sort<K, HV, BLOCK, IPT, WLEV>), dim3((unsigned)grid), dim3(BLOCK), 0, s, in, out, n,
-                       cin, cout, vin, vout);
+                       cin, cout, vin, vout, sp ? *sp : none);
     return hipGetLastError();
 }
 

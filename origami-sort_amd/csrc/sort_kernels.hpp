@@ -53,10 +53,24 @@ hipError_t launch_expand(const SegRecH *segs, uint64_t nseg, TileDescH *td0, Til
                          MergeDescH *md, const uint64_t *poff, uint64_t nlarge, uint64_t LT, uint64_t MT, uint32_t w,
                          hipStream_t s);
 
+// Samples for a sample-split K-way pass: every 2^slog2-th item of each output
+// run, tagged with the index of its run within the next pass's group of `kw`
+// runs of 2^rlog2 items and with its index in that run over 2^slog2.  32-bit
+// keys are stored as u64 key << 32 | run << 28 | index (so u64 order is the
+// total order (key, run, index)); 64-bit keys as {key, run << 32 | index}
+// records.  smp == nullptr: no samples.
+struct SampleSpec {
+    void *smp;
+    int rlog2;
+    int kw;
+    int slog2;
+};
+
 // vin / vout (KV only; else nullptr): values read from or written to a separate
-// array, keys from `in` / to `out` (structure of arrays).
+// array, keys from `in` / to `out` (structure of arrays).  sp (nullable, key-only
+// classes): samples of the sorted tiles for the first K-way pass.
 typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s,
-                                  const void *vin, void *vout);
+                                  const void *vin, void *vout, const SampleSpec *sp);
 typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s);
 // part == nullptr: co-ranks found inside the merge kernel; else a table of
 // ceil(n / tile) entries that the partition kernel fills first.
@@ -102,6 +116,25 @@ hipError_t launch_generate(int width, int dist, uint64_t seed, uint64_t n, void 
 hipError_t launch_generate_tagged(uint64_t seed, uint64_t n, int bits, const uint64_t *starts, uint64_t nb,
                                   void *pairs, hipStream_t s);
 hipError_t launch_generate_untag(int width, const void *pairs, uint64_t n, void *out, hipStream_t s);
+
+// One sample-split K-way pass over runs of 2^rlog2 items (kw of them per
+// group): the split kernel reads the unmerged samples s0 and the merged
+// samples sm (both spaced 2^slog2) and fills tab (kway_tiles * kw entries),
+// then the merge kernel writes the output and, when `next` names a buffer, the
+// samples of the output runs for the pass after this one.  `mid` is recorded
+// between the two kernels.
+typedef hipError_t (*KwayLaunch)(const void *in, void *out, uint64_t n, int rlog2, int slog2, int cout,
+                                 uint64_t *tab, const void *s0, const void *sm, const SampleSpec *next, hipStream_t s,
+                                 hipEvent_t mid);
+struct KwayShapeFns {
+    int block, ipt;
+    KwayLaunch run[4];  // fan-in 2, 4, 8, 16
+    uint64_t tile() const { return (uint64_t)block * ipt; }
+};
+// Tiles of one K-way pass (0 when the shape cannot hold kw slices at this
+// sample spacing).
+uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int ipt);
+const KwayShapeFns *kway_shapes(KeyClass kc, int *count);
 
 const BlockShapeFns *block_shapes(KeyClass kc, int *count);
 const MergeShapeFns *merge_shapes(KeyClass kc, int *count);

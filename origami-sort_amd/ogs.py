@@ -13,6 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("OGS_LIB") or os.path.join(HERE, "libogs.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ogs.h")
+DEFAULT_TILES = os.path.join(HERE, "ogs_tiles.txt")  # the tile table every new context loads
 
 U32, I32, U64, I64, F32, KV_U64U64, KV_I64I64 = range(7)
 KINDS = {"u32": U32, "i32": I32, "u64": U64, "i64": I64, "f32": F32,

@@ -1,0 +1,157 @@
+"""GPU parity of the sample-split K-way merge pass (kway_kernels.hip) against the
+oracle: key-only kinds, every fan-in (2, 4, 8, 16), sample spacings, tile shapes,
+distributions with heavy ties, ragged sizes, and a large size checked against an
+independent device sort of the same keys."""
+import sys
+
+import numpy as np
+import pytest
+
+import ogs
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+KEY_ONLY = (ogs.U32, ogs.I32, ogs.F32, ogs.U64, ogs.I64)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    c = ogs.Context(0)
+    yield c
+    c.close()
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def configure(ctx, kind, fanin, sample=5, shape=0, small_block=True):
+    # small block tiles give many merge levels at test sizes
+    if small_block:
+        ctx.tune_set(kind, "plain_block", 10 if ogs.WIDTH[kind] == 4 else 1)  # 256 x 8 keys
+    ctx.tune_set(kind, "kway_fanin", fanin)
+    ctx.tune_set(kind, "kway_sample", sample)
+    ctx.tune_set(kind, "kway_shape", shape)
+
+
+def restore(ctx, kind):
+    ctx.tune_load(ogs.DEFAULT_TILES)
+
+
+def gpu_sort(ctx, kind, a):
+    import torch
+    n = a.size
+    d = ogs.to_device(a)
+    s = torch.empty_like(d)
+    res = ctx.sort_tensor(kind, d, s, n)
+    torch.cuda.synchronize()
+    return ogs.from_device(res, kind, n)
+
+
+def check(kind, got, a):
+    ref = oracle.sort(kind, a)
+    assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("kind", KEY_ONLY)
+@pytest.mark.parametrize("fanin", [2, 4, 8, 16])
+@pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.ZIPF, ogs.SORTED, ogs.REVERSE, ogs.DISTINCT16])
+def test_kway_kinds(ctx, kind, fanin, dist):
+    n = 300007
+    say(f"case kway kind={ogs.KIND_NAMES[kind]} fanin={fanin} dist={dist} n={n}")
+    configure(ctx, kind, fanin)
+    try:
+        a = ogs.generate(kind, ogs.NORMAL if (kind == ogs.F32 and dist == ogs.UNIFORM) else dist, n, seed=11 + fanin)
+        check(kind, gpu_sort(ctx, kind, a), a)
+    finally:
+        restore(ctx, kind)
+
+
+@pytest.mark.parametrize("n", [2049, 4096, 4097, 16385, 65536, 65537, 262143, 1000003])
+@pytest.mark.parametrize("fanin", [4, 16])
+def test_kway_sizes(ctx, n, fanin):
+    say(f"case kway u32 sizes fanin={fanin} n={n}")
+    configure(ctx, ogs.U32, fanin)
+    try:
+        a = ogs.generate(ogs.U32, ogs.UNIFORM, n, seed=n)
+        check(ogs.U32, gpu_sort(ctx, ogs.U32, a), a)
+    finally:
+        restore(ctx, ogs.U32)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64])
+@pytest.mark.parametrize("sample", [3, 4, 6, 7])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+def test_kway_shapes_and_spacing(ctx, kind, sample, shape):
+    n = 777777
+    say(f"case kway kind={ogs.KIND_NAMES[kind]} sample={sample} shape={shape} n={n}")
+    configure(ctx, kind, 16, sample=sample, shape=shape, small_block=False)
+    try:
+        a = ogs.generate(kind, ogs.DISTINCT16, n, seed=sample * 10 + shape)
+        check(kind, gpu_sort(ctx, kind, a), a)
+        b = ogs.generate(kind, ogs.UNIFORM, n, seed=sample * 10 + shape)
+        check(kind, gpu_sort(ctx, kind, b), b)
+    finally:
+        restore(ctx, kind)
+
+
+def test_kway_extremes(ctx):
+    # maximum keys (the pad value) mixed with ordinary keys, all-equal runs
+    n = 200003
+    rng = np.random.default_rng(3)
+    for kind in (ogs.U32, ogs.U64):
+        say(f"case kway extremes kind={ogs.KIND_NAMES[kind]}")
+        configure(ctx, kind, 16)
+        try:
+            dt = np.uint32 if kind == ogs.U32 else np.uint64
+            mx = np.iinfo(dt).max
+            a = rng.choice(np.array([0, 1, mx - 1, mx], dtype=dt), n)
+            check(kind, gpu_sort(ctx, kind, a), a)
+            a = np.full(n, mx, dtype=dt)
+            check(kind, gpu_sort(ctx, kind, a), a)
+        finally:
+            restore(ctx, kind)
+
+
+def test_kway_keep_input_and_pass_count(ctx):
+    import torch
+    n = (1 << 20) + 3
+    configure(ctx, ogs.U32, 16, small_block=False)
+    try:
+        a = ogs.generate(ogs.U32, ogs.UNIFORM, n, seed=9)
+        d_in = ogs.to_device(a)
+        d0, d1 = torch.empty_like(d_in), torch.empty_like(d_in)
+        res = ctx.sort_keep_ptr(ogs.U32, d_in.data_ptr(), d0.data_ptr(), d1.data_ptr(), n)
+        torch.cuda.synchronize()
+        assert res == d0.data_ptr()  # the result lands in the data buffer
+        check(ogs.U32, ogs.from_device(d0, ogs.U32, n), a)
+        assert np.array_equal(ogs.from_device(d_in, ogs.U32, n), a)
+        T = 512 * 32
+        levels = int(np.ceil(np.log2(n / T)))
+        assert ctx.pass_count(ogs.U32, n) == 1 + (levels + 3) // 4
+    finally:
+        restore(ctx, ogs.U32)
+
+
+def test_kway_large_checksums(ctx):
+    import torch
+    n = (1 << 27) + 12345
+    say(f"case kway u32 large n={n}")
+    configure(ctx, ogs.U32, 16, small_block=False)
+    try:
+        g = torch.Generator(device="cuda").manual_seed(5)
+        d = torch.randint(-2**31, 2**31, (n,), dtype=torch.int32, device="cuda", generator=g)
+        # an independent sort of the same keys (the unsigned order as int64)
+        ref = torch.sort(d.to(torch.int64) & 0xFFFFFFFF).values
+        s = torch.empty_like(d)
+        out = ctx.sort_tensor(ogs.U32, d, s, n)
+        torch.cuda.synchronize()
+        o = out.view(torch.int32)[:n].to(torch.int64) & 0xFFFFFFFF
+        assert bool((o[1:] >= o[:-1]).all())
+        assert bool(torch.equal(o, ref))
+    finally:
+        restore(ctx, ogs.U32)

@@ -1,0 +1,260 @@
+"""CPU model of the sample-split K-way merge pass (origami-sort_amd/csrc/kway_kernels.hip).
+
+The model follows the kernels step by step -- tagged samples every s items of each
+run, the per-group sample merge, the two-stage split search of k_kway_bounds, and
+k_kway_merge's padded LDS layout (odd slices descending), per-thread co-ranks on
+the bitonic layout and level-by-level pair merges -- and checks the invariants
+the pass relies on: splits are monotone and exact (every item of a tile precedes
+every item of the next in the total order (key, run, index)), every tile with
+its padded slices fits the LDS tile, and the tiles laid end to end are the
+merge of the group.  This is the role of Partition_Ptrs (Origami/Partition.h:
+117-166) for KW runs, restated on the tiles of one pass.
+"""
+import numpy as np
+import pytest
+
+
+def kw_q(C, ipt, kw, slog2):
+    q = (C - kw * ipt) // (1 << slog2) - kw
+    return max(q, 0)
+
+
+def group_samples(runs, slog2):
+    """Tagged samples (key, run, index/s) of each run, merged in the total order."""
+    s = 1 << slog2
+    tags = []
+    for j, X in enumerate(runs):
+        for m in range(0, len(X), s):
+            tags.append((int(X[m]), j, m >> slog2))
+    tags.sort()
+    return tags
+
+
+def split(X, v, up, slog2, samples_j):
+    """k_kway_bounds for one run: the items of X that precede the boundary."""
+    s = 1 << slog2
+    lo, hi = 0, len(samples_j)
+    while lo < hi:
+        mid = (lo + hi) // 2
+        k = samples_j[mid]
+        if (not (v < k)) if up else (k < v):
+            lo = mid + 1
+        else:
+            hi = mid
+    if lo == 0:
+        return 0
+    a, b = ((lo - 1) << slog2) + 1, min(lo << slog2, len(X))
+    while a < b:
+        mid = (a + b) // 2
+        k = int(X[mid])
+        if (not (v < k)) if up else (k < v):
+            a = mid + 1
+        else:
+            b = mid
+    return a
+
+
+def corank_rev(L, a_off, la, b_top, lb, diag):
+    """lds_corank_lift_rev: A items among the first diag outputs (ties to A)."""
+    lo = max(0, diag - lb)
+    hi = min(diag, la)
+    top = 1 << (min(la, lb).bit_length() - 1) if min(la, lb) > 0 else 0
+    a = lo
+    s = top
+    while s > 0:
+        t = min(a + s, hi)
+        ka = L[a_off + t - 1]
+        kb = L[b_top - diag + t]
+        a = a if kb < ka else t
+        s >>= 1
+    return a
+
+
+def merge_rev(L, ia, ib, ipt):
+    """lds_merge_rev: A moving up from ia, B moving down from ib."""
+    out = []
+    ka, kb = L[ia], L[ib]
+    for _ in range(ipt):
+        take_a = not (kb < ka)
+        out.append(ka if take_a else kb)
+        if take_a:
+            ia += 1
+            ka = L[ia]
+        else:
+            ib -= 1
+            kb = L[ib]
+    return out
+
+
+KMAX = 2**32 - 1
+
+
+def model_tile(runs, lo, hi, block, ipt):
+    """k_kway_merge on one tile: returns the tile's output (first cnt items)."""
+    kw = len(runs)
+    C = block * ipt
+    n = [hi[j] - lo[j] for j in range(kw)]
+    assert all(x >= 0 for x in n)
+    P = [0]
+    for j in range(kw):
+        P.append(P[-1] + (n[j] + ipt - 1) // ipt * ipt)
+    Np = P[-1]
+    assert Np <= C, (Np, C)
+    # LDS starts as garbage (guard reads such as A[-1] happen only where they
+    # cannot change the result; the output check catches any that would)
+    L = [int(x) for x in np.random.default_rng(Np).integers(0, 2**32, C + 2 * ipt + 2)]
+    # load phase: chunk table per thread, then every LDS position e
+    cb, cm = {}, {}
+    for tid in range(block):
+        c0 = tid * ipt
+        if c0 >= Np:
+            continue
+        j0 = sum(1 for j in range(1, kw) if P[j] <= c0)
+        pj, npj, nj = P[j0], P[j0 + 1] - P[j0], n[j0]
+        q = c0 - pj
+        desc = j0 & 1
+        if desc:
+            base = lo[j0] + (npj - 1 - q)
+            v0, v1 = min(max(npj - q - nj, 0), ipt), ipt
+        else:
+            base = lo[j0] + q
+            v0, v1 = 0, min(max(nj - q, 0), ipt)
+        cb[tid] = (j0, base, desc)
+        cm[tid] = (v0, v1)
+    for e in range(Np):
+        c, r = e // ipt, e % ipt
+        j0, base, desc = cb[c]
+        v0, v1 = cm[c]
+        if v0 <= r < v1:
+            L[e] = int(runs[j0][base - r if desc else base + r])
+        else:
+            L[e] = KMAX
+    logkw = kw.bit_length() - 1
+    for lev in range(1, logkw + 1):
+        regs = {}
+        for tid in range(block):
+            c0 = tid * ipt
+            if c0 >= Np:
+                continue
+            j0 = sum(1 for j in range(1, kw) if P[j] <= c0)
+            hv = 1 << (lev - 1)
+            s0 = (j0 >> lev) << lev
+            a_off, mid, end = P[s0], P[s0 + hv], P[s0 + 2 * hv]
+            la, lb, diag = mid - a_off, end - mid, c0 - a_off
+            a = corank_rev(L, a_off, la, end - 1, lb, diag)
+            regs[tid] = (merge_rev(L, a_off + a, end - 1 - (diag - a), ipt), j0, end, diag)
+        for tid, (r, j0, end, diag) in regs.items():
+            c0 = tid * ipt
+            if lev < logkw and ((j0 >> lev) & 1):
+                top = end - 1 - diag
+                for i in range(ipt):
+                    L[top - i] = r[i]
+            else:
+                for i in range(ipt):
+                    L[c0 + i] = r[i]
+    cnt = sum(n)
+    return L[:cnt]
+
+
+def model_pass(keys, R, kw, slog2, block, ipt):
+    """One K-way pass over keys (runs of R sorted): returns the output."""
+    nitems = len(keys)
+    C = block * ipt
+    Q = kw_q(C, ipt, kw, slog2)
+    assert Q > 0
+    out = np.empty_like(keys)
+    for gbase in range(0, nitems, kw * R):
+        runs = [keys[gbase + j * R: min(gbase + (j + 1) * R, nitems)] if gbase + j * R < nitems
+                else keys[:0] for j in range(kw)]
+        for X in runs:
+            assert np.all(X[1:] >= X[:-1])
+        tags = group_samples(runs, slog2)
+        ns = len(tags)
+        nt = (ns + Q - 1) // Q
+        smp = [[int(X[m]) for m in range(0, len(X), 1 << slog2)] for X in runs]
+        bounds = []
+        for tl in range(nt):
+            if tl == 0:
+                bounds.append([0] * kw)
+                continue
+            v, r, m = tags[tl * Q]
+            row = []
+            for j in range(kw):
+                if len(runs[j]) == 0:
+                    row.append(0)
+                elif j == r:
+                    row.append(m << slog2)
+                else:
+                    row.append(split(runs[j], v, j < r, slog2, smp[j]))
+            bounds.append(row)
+        bounds.append([len(X) for X in runs])
+        o = 0
+        for tl in range(nt):
+            lo, hi = bounds[tl], bounds[tl + 1]
+            assert all(lo[j] <= hi[j] for j in range(kw)), (lo, hi)
+            # exactness: every item of the tile precedes every item after it
+            if tl + 1 < nt:
+                pre = [(int(runs[j][i]), j, i) for j in range(kw) for i in range(lo[j], hi[j])]
+                post = [(int(runs[j][hi[j]]), j, hi[j]) for j in range(kw) if hi[j] < len(runs[j])]
+                if pre and post:
+                    assert max(pre) < min(post)
+            size = sum(hi[j] - lo[j] for j in range(kw))
+            assert size <= (Q + kw) * (1 << slog2) - kw
+            assert sum(lo) == o
+            tile = model_tile(runs, lo, hi, block, ipt)
+            out[gbase + o: gbase + o + size] = tile
+            o += size
+        assert o == sum(len(X) for X in runs)
+    return out
+
+
+def runs_of(keys, R):
+    k = keys.copy()
+    for b in range(0, len(k), R):
+        k[b:b + R] = np.sort(k[b:b + R])
+    return k
+
+
+@pytest.mark.parametrize("kw", [2, 4, 8, 16])
+@pytest.mark.parametrize("dist", ["uniform", "few", "sorted", "reverse"])
+def test_kway_pass_model(kw, dist):
+    rng = np.random.default_rng(kw * 7 + len(dist))
+    block, ipt, slog2 = 64, 8, 3
+    R = 1024
+    n = R * kw * 2 + 3 * R + 77  # two full groups and a ragged one
+    if dist == "uniform":
+        keys = rng.integers(0, 2**32, n, dtype=np.uint64)
+    elif dist == "few":
+        keys = rng.integers(0, 4, n, dtype=np.uint64) * 1000
+    elif dist == "sorted":
+        keys = np.arange(n, dtype=np.uint64) // 3
+    else:
+        keys = (n - np.arange(n, dtype=np.uint64)) // 5
+    keys = runs_of(keys, R)
+    out = model_pass(keys, R, kw, slog2, block, ipt)
+    ref = np.concatenate([np.sort(keys[b:b + kw * R]) for b in range(0, n, kw * R)])
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("kw,ipt,slog2", [(16, 8, 2), (4, 16, 4), (8, 4, 3)])
+def test_kway_pass_model_shapes(kw, ipt, slog2):
+    rng = np.random.default_rng(5)
+    block = 1024 // ipt
+    R = 512
+    n = R * kw + R // 2 + 5
+    keys = runs_of(rng.integers(0, 50, n, dtype=np.uint64), R)
+    out = model_pass(keys, R, kw, slog2, block, ipt)
+    ref = np.concatenate([np.sort(keys[b:b + kw * R]) for b in range(0, n, kw * R)])
+    assert np.array_equal(out, ref)
+
+
+def test_kway_tile_bounds_hold_at_headline_shapes():
+    """Q leaves room for the padded slices at the shapes the library registers."""
+    for C, ipt in ((16384, 32), (8192, 32), (8192, 16), (4096, 16), (4096, 8), (2048, 8)):
+        for kw in (2, 4, 8, 16):
+            for slog2 in range(3, 8):
+                q = kw_q(C, ipt, kw, slog2)
+                if q == 0:
+                    continue
+                worst = (q + kw) * (1 << slog2) - kw + kw * (ipt - 1)
+                assert worst <= C
