@@ -140,10 +140,10 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
     __shared__ uint64_t s_src[KW];  // global index of each slice's first item
     __shared__ int s_n[KW];         // items per slice
     __shared__ int s_P[KW + 1];     // prefix sums of the padded slice lengths
-    __shared__ uint64_t s_cb[BLOCK];  // per chunk of IPT LDS items: source of its item 0
-    __shared__ int s_cm[BLOCK];       // per chunk: first and end real position, direction
+    __shared__ uint64_t s_o0;       // output index of the tile's first item
+    __shared__ int s_cnt;           // items in the tile (-1: inconsistent table)
     K *sk = skb + lds_front<K>();
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63;
     const uint64_t t = blockIdx.x;
     const uint64_t R = 1ull << rlog2;
     const uint64_t g = t / ntg, tl = t % ntg;
@@ -158,49 +158,46 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
         s_n[tid] = hi >= lo && hi <= lenj ? (int)(hi - lo) : -1;
     }
     __syncthreads();
-    int P[KW + 1];
-    int cnt = 0;
-    uint64_t o0 = gbase;
-    bool bad = false;
-    P[0] = 0;
-#pragma unroll
-    for (int j = 0; j < KW; j++) {
-        const int nj = s_n[j];
-        bad |= nj < 0;
-        cnt += nj;
-        P[j + 1] = P[j] + ((nj + IPT - 1) & ~(IPT - 1));
-        o0 += s_src[j] - (gbase + (uint64_t)j * R);
-    }
-    const int Np = P[KW];
-    if (bad || Np > C) return;  // uniform: cannot happen with a consistent table
     if (tid == 0) {
-#pragma unroll
-        for (int j = 0; j <= KW; j++) s_P[j] = P[j];
+        int p = 0, cnt = 0;
+        bool bad = false;
+        uint64_t o0 = gbase;
+        s_P[0] = 0;
+        for (int j = 0; j < KW; j++) {
+            const int nj = s_n[j];
+            bad |= nj < 0;
+            cnt += nj;
+            p += (nj + IPT - 1) & ~(IPT - 1);
+            s_P[j + 1] = p;
+            o0 += s_src[j] - (gbase + (uint64_t)j * R);
+        }
+        s_o0 = o0;
+        s_cnt = bad || p > C ? -1 : cnt;  // cannot happen with a consistent table
     }
-    // this thread's chunk [c0, c0 + IPT) lies in slice j0
+    __syncthreads();
+    const int cnt = s_cnt;
+    if (cnt < 0) return;  // uniform
+    const int Np = s_P[KW];
+    const uint64_t o0 = s_o0;
+    // this thread's chunk [c0, c0 + IPT) lies in slice j0: the last j with
+    // s_P[j] <= c0 (binary search over the prefix sums)
     const int c0 = tid * IPT;
     const bool live = c0 < Np;
     int j0 = 0;
 #pragma unroll
-    for (int j = 1; j < KW; j++) j0 += P[j] <= c0 ? 1 : 0;
+    for (int b = KW / 2; b >= 1; b >>= 1) j0 += s_P[j0 + b] <= c0 ? b : 0;
+    // chunk source: position c0 + r holds item base + r (ascending slices) or
+    // base - r (odd slices, the B side of the first level, stored descending);
+    // real items are r in [v0, v1), the rest are pads
+    uint32_t cb_lo = 0, cb_hi = 0, cmeta = 0;
     if (live) {
-        // odd slices are the B side of the first level: stored descending
-        int pj = 0, npj = 0, nj = 0;
-        uint64_t src = 0;
-#pragma unroll
-        for (int j = 0; j < KW; j++)
-            if (j == j0) {
-                pj = P[j];
-                npj = P[j + 1] - P[j];
-                nj = s_n[j];
-                src = s_src[j];
-            }
+        const int pj = s_P[j0], npj = s_P[j0 + 1] - pj, nj = s_n[j0];
+        const uint64_t src = s_src[j0];
         const int q = c0 - pj;
         const bool desc = (j0 & 1) != 0;
         int v0, v1;
         uint64_t base;
         if (desc) {
-            // position q + r holds item npj - 1 - q - r, real when < nj
             base = src + (uint64_t)(npj - 1 - q);
             v0 = npj - q - nj;
             v0 = v0 < 0 ? 0 : v0 > IPT ? IPT : v0;
@@ -211,31 +208,39 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
             v1 = nj - q;
             v1 = v1 < 0 ? 0 : v1 > IPT ? IPT : v1;
         }
-        s_cb[tid] = base;
-        s_cm[tid] = v0 | (v1 << 8) | ((int)desc << 16);
+        cb_lo = (uint32_t)base;
+        cb_hi = (uint32_t)(base >> 32);
+        cmeta = (uint32_t)(v0 | (v1 << 8) | ((int)desc << 16));
     }
-    __syncthreads();
     {
-        // all loads in flight at once; pads and positions past the tile read
-        // the group's first item and keep the maximum key
+        // wave w loads LDS positions [w * 64 * IPT, (w + 1) * 64 * IPT): the
+        // chunks of its own lanes; load i covers positions 64 * i + lane of that
+        // range, in chunk i * (64 / IPT) + lane / IPT, whose source the lane
+        // fetches from the owning lane.  All loads are in flight at once; pads
+        // read the group's first item and keep the maximum key.
+        constexpr int CPI = 64 / IPT;
+        const int r = lane & (IPT - 1);
+        const int wbase = (tid >> 6) * 64 * IPT;
         K kk[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
-            const int e = i * BLOCK + tid;
-            const int ec = e < Np ? e : 0;
-            const int c = ec / IPT, r = ec % IPT;
-            const uint64_t b = s_cb[c];
-            const int m = s_cm[c];
-            const bool real = e < Np && r >= (m & 255) && r < ((m >> 8) & 255);
-            const uint64_t src = real ? ((m >> 16) ? b - (uint64_t)r : b + (uint64_t)r) : gbase;
+            const int sl = (i * CPI + lane / IPT) << 2;
+            const uint32_t blo = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_lo);
+            const uint32_t bhi = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_hi);
+            const uint32_t m = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cmeta);
+            const bool real = r >= (int)(m & 255) && r < (int)((m >> 8) & 255);
+            // branch-free: offset +r or -r (conditional negate by the sign
+            // mask), then one select of the address
+            const int neg = -(int)(m >> 16);
+            const int64_t off = (int64_t)((r ^ neg) - neg);
+            const uint64_t b = ((uint64_t)bhi << 32) | blo;
+            const uint64_t mask = (uint64_t)0 - (uint64_t)real;
+            const uint64_t src = gbase + ((b + (uint64_t)off - gbase) & mask);
             const K x = in[src];
             kk[i] = real ? x : KMAX;
         }
 #pragma unroll
-        for (int i = 0; i < IPT; i++) {
-            const int e = i * BLOCK + tid;
-            if (e < Np) sk[lsw<K>(e)] = kk[i];
-        }
+        for (int i = 0; i < IPT; i++) sk[lsw<K>(wbase + 64 * i + lane)] = kk[i];
     }
     __syncthreads();
     K r[IPT];
@@ -260,8 +265,14 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
         }
         __syncthreads();
     }
+    K *op = out + o0;
+    if (cout == 0) {
 #pragma unroll 4
-    for (int e = tid; e < cnt; e += BLOCK) out[o0 + e] = dec_key(sk[lsw<K>(e)], cout);
+        for (int e = tid; e < cnt; e += BLOCK) op[e] = sk[lsw<K>(e)];
+    } else {
+#pragma unroll 4
+        for (int e = tid; e < cnt; e += BLOCK) op[e] = dec_key(sk[lsw<K>(e)], cout);
+    }
     if (sp.smp) write_samples<BLOCK>(sk, cnt, o0, sp.smp, sp.rlog2, sp.kw, sp.slog2);
 }
 

@@ -380,6 +380,23 @@ __device__ __forceinline__ int lds_corank_lift_rev(const K *sk, int a_off, int l
 template <int IPT, bool FULL, typename K>
 __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i0, int cnt, K (&r)[IPT]) {
     K ka = sk[lsw<K>(ia)], kb = sk[lsw<K>(ib)];
+    if constexpr (FULL) {
+        // One pointer: after i + 1 outputs the B pointer is ib0 - (i + 1 - (ia -
+        // ia0)) = ia + dlt - (i + 1), with dlt = ib0 - ia0, so each step updates
+        // ia alone and picks the next LDS index as ia or ia + dlt - (i + 1).
+        const int dlt = ib - ia;
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const bool takeA = !(kb < ka);
+            r[i] = takeA ? ka : kb;
+            ia += takeA ? 1 : 0;
+            const int nb = ia + dlt - (i + 1);
+            const K nk = sk[lsw<K>(takeA ? ia : nb)];
+            ka = takeA ? nk : ka;
+            kb = takeA ? kb : nk;
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < IPT; i++) {
         if (FULL || (i >= i0 && i < i0 + cnt)) {
