@@ -124,6 +124,17 @@ __global__ void k_kway_bounds(const K *__restrict__ in, uint64_t n, int rlog2, i
     tab[t * KW + j] = split;
 }
 
+// Items tid + k * BLOCK (k < IPT) of an LDS tile (sp0 = its swizzled base) to
+// ob[tid + k * BLOCK], with output codec CODEC: all reads, then all stores.
+template <int BLOCK, int IPT, int CODEC, typename K>
+__device__ __forceinline__ void store_items(const K *sp0, K *ob, int tid) {
+    K v[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; k++) v[k] = dec_key(sp0[k * BLOCK], CODEC);
+#pragma unroll
+    for (int k = 0; k < IPT; k++) ob[(uint32_t)(tid + k * BLOCK)] = v[k];
+}
+
 // ----------------------------------------------------------------------------
 // The merge: one workgroup per tile.
 // ----------------------------------------------------------------------------
@@ -142,6 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
     __shared__ int s_P[KW + 1];     // prefix sums of the padded slice lengths
     __shared__ uint64_t s_o0;       // output index of the tile's first item
     __shared__ int s_cnt;           // items in the tile (-1: inconsistent table)
+    __shared__ int s_top[LOGKW + 1];  // per level: co-rank step bound over all pairs
     K *sk = skb + lds_front<K>();
     const int tid = threadIdx.x, lane = tid & 63;
     const uint64_t t = blockIdx.x;
@@ -173,6 +185,18 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
         }
         s_o0 = o0;
         s_cnt = bad || p > C ? -1 : cnt;  // cannot happen with a consistent table
+        // the co-rank of a pair (la, lb) needs pow2_floor(min(la, lb)) lifting
+        // steps; every thread of a level takes the largest, so the step loop
+        // is uniform (more steps than a pair needs are harmless)
+        for (int lev = 1; lev <= LOGKW; lev++) {
+            int mx = 0;
+            for (int s0 = 0; s0 < KW; s0 += 2 << (lev - 1)) {
+                const int hv = 1 << (lev - 1);
+                const int la = s_P[s0 + hv] - s_P[s0], lb = s_P[s0 + 2 * hv] - s_P[s0 + hv];
+                mx = max(mx, min(la, lb));
+            }
+            s_top[lev] = pow2_floor(mx);
+        }
     }
     __syncthreads();
     const int cnt = s_cnt;
@@ -218,13 +242,19 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
         // range, in chunk i * (64 / IPT) + lane / IPT, whose source the lane
         // fetches from the owning lane.  All loads are in flight at once; pads
         // read the group's first item and keep the maximum key.
-        constexpr int CPI = 64 / IPT;
+        // Load i covers positions wbase + BLK * (i % NB) + 64 * (i / NB) + lane,
+        // BLK = the index span the LDS swizzle repeats over, so a lane's LDS
+        // index is its group's swizzled base plus a multiple of BLK.
+        constexpr int BLK = sizeof(K) == 4 ? 256 : 128;
+        constexpr int NB = 64 * IPT / BLK;
+        static_assert(NB >= 1 && 64 * IPT % BLK == 0, "a wave's load range is whole swizzle blocks");
         const int r = lane & (IPT - 1);
         const int wbase = (tid >> 6) * 64 * IPT;
         K kk[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
-            const int sl = (i * CPI + lane / IPT) << 2;
+            const int pw = BLK * (i % NB) + 64 * (i / NB) + lane;  // position within the wave's range
+            const int sl = (pw / IPT) << 2;
             const uint32_t blo = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_lo);
             const uint32_t bhi = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_hi);
             const uint32_t m = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cmeta);
@@ -240,7 +270,8 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
             kk[i] = real ? x : KMAX;
         }
 #pragma unroll
-        for (int i = 0; i < IPT; i++) sk[lsw<K>(wbase + 64 * i + lane)] = kk[i];
+        for (int i = 0; i < IPT; i++)
+            sk[lsw<K>(wbase + 64 * (i / NB) + lane) + BLK * (i % NB)] = kk[i];
     }
     __syncthreads();
     K r[IPT];
@@ -255,7 +286,8 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
             end = s_P[s0 + 2 * hv];
             const int la = mid - a_off, lb = end - mid;
             diag = c0 - a_off;
-            lds_merge_slice_rev<IPT>(sk, a_off, la, end - 1, lb, diag, IPT, pow2_floor(la < lb ? la : lb), r);
+            const int top = __builtin_amdgcn_readfirstlane(s_top[lev]);
+            lds_merge_slice_rev<IPT>(sk, a_off, la, end - 1, lb, diag, IPT, top, r);
         }
         __syncthreads();
         if (live) {
@@ -265,13 +297,26 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
         }
         __syncthreads();
     }
-    K *op = out + o0;
-    if (cout == 0) {
-#pragma unroll 4
-        for (int e = tid; e < cnt; e += BLOCK) op[e] = sk[lsw<K>(e)];
+    // item tid + k * BLOCK sits at LDS index lsw(tid) + k * BLOCK: BLOCK is a
+    // multiple of the span the swizzle repeats over
+    static_assert(BLOCK % (sizeof(K) == 4 ? 256 : 128) == 0, "BLOCK: whole swizzle blocks");
+    // o0 is the same in every lane: kept in scalar registers, the stores take
+    // it as their base and a 32-bit lane offset
+    const uint64_t o0u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(o0 >> 32)) << 32) |
+                         (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o0);
+    K *ob = out + o0u;
+    const K *sp0 = sk + lsw<K>(tid);
+    if (tid + (IPT - 1) * BLOCK < cnt) {
+        // all IPT items of this thread are real: every read, then every store
+        // (the codec is uniform: one branch picks the loop)
+        if (cout == 0) store_items<BLOCK, IPT, 0>(sp0, ob, tid);
+        else if (cout == 1) store_items<BLOCK, IPT, 1>(sp0, ob, tid);
+        else store_items<BLOCK, IPT, 2>(sp0, ob, tid);
     } else {
-#pragma unroll 4
-        for (int e = tid; e < cnt; e += BLOCK) op[e] = dec_key(sk[lsw<K>(e)], cout);
+        const int nk = cnt > tid ? (cnt - tid + BLOCK - 1) / BLOCK : 0;
+#pragma unroll
+        for (int k = 0; k < IPT; k++)
+            if (k < nk) ob[(uint32_t)(tid + k * BLOCK)] = dec_key(sp0[k * BLOCK], cout);
     }
     if (sp.smp) write_samples<BLOCK>(sk, cnt, o0, sp.smp, sp.rlog2, sp.kw, sp.slog2);
 }

@@ -34,6 +34,31 @@ __device__ __forceinline__ uint32_t dec_key(uint32_t x, int c) {
 __device__ __forceinline__ uint64_t enc_key(uint64_t x, int c) { return c == 1 ? x ^ (1ull << 63) : x; }
 __device__ __forceinline__ uint64_t dec_key(uint64_t x, int c) { return c == 1 ? x ^ (1ull << 63) : x; }
 
+// The same codecs without branches, for loops over many keys with one codec:
+// enc(x) = x ^ ((sx & a) | b), dec(x) = x ^ ((~sx & a) | b), with sx all ones
+// when the top bit of x is set; (a, b) = (0, 0) identity, (0, top) sign flip,
+// (~0, top) float32 totalOrder.
+template <typename K> struct CodecMask {
+    K a, b;
+};
+template <typename K> __device__ __forceinline__ CodecMask<K> codec_mask(int c) {
+    constexpr K TOP = (K)1 << (sizeof(K) * 8 - 1);
+    CodecMask<K> m;
+    m.a = (sizeof(K) == 4 && c == 2) ? (K)~(K)0 : (K)0;
+    m.b = c != 0 ? TOP : (K)0;
+    return m;
+}
+template <typename K> __device__ __forceinline__ K sign_all(K x) {
+    if constexpr (sizeof(K) == 4) return (K)((int32_t)x >> 31);
+    else return (K)((int64_t)x >> 63);
+}
+template <typename K> __device__ __forceinline__ K enc_bf(K x, CodecMask<K> m) {
+    return x ^ ((sign_all(x) & m.a) | m.b);
+}
+template <typename K> __device__ __forceinline__ K dec_bf(K x, CodecMask<K> m) {
+    return x ^ ((~sign_all(x) & m.a) | m.b);
+}
+
 template <typename K> struct KeyMax;
 template <> struct KeyMax<uint32_t> { static constexpr uint32_t v = 0xFFFFFFFFu; };
 template <> struct KeyMax<uint64_t> { static constexpr uint64_t v = ~0ull; };
@@ -389,9 +414,11 @@ __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i
         for (int i = 0; i < IPT; i++) {
             const bool takeA = !(kb < ka);
             r[i] = takeA ? ka : kb;
-            ia += takeA ? 1 : 0;
+            const int ia1 = ia + 1;
             const int nb = ia + dlt - (i + 1);
-            const K nk = sk[lsw<K>(takeA ? ia : nb)];
+            const int nx = takeA ? ia1 : nb;
+            ia = takeA ? ia1 : ia;
+            const K nk = sk[lsw<K>(nx)];
             ka = takeA ? nk : ka;
             kb = takeA ? kb : nk;
         }
@@ -517,20 +544,26 @@ __device__ __forceinline__ void lds_store_tile(const K *sk, const uint64_t *sv, 
         K *o = (K *)out + dst;
         const bool al16 = ((uintptr_t)o & 15) == 0;
         if (al16 && cnt == TILE) {
+            // uniform codec: no per-key branch; stores off one base pointer
+            const CodecMask<K> m = codec_mask<K>(cout);
 #pragma unroll
             for (int q = tid; q < TILE / E; q += BLOCK) {
-                const int e = q * E;
+                const uint32_t e = (uint32_t)(q * E);
                 if constexpr (E == 4) {
                     uint4 x = *(const uint4 *)&sk[lsw<K>(e)];
-                    x.x = (uint32_t)dec_key((uint32_t)x.x, cout);
-                    x.y = (uint32_t)dec_key((uint32_t)x.y, cout);
-                    x.z = (uint32_t)dec_key((uint32_t)x.z, cout);
-                    x.w = (uint32_t)dec_key((uint32_t)x.w, cout);
+                    if (cout) {
+                        x.x = dec_bf<uint32_t>(x.x, m);
+                        x.y = dec_bf<uint32_t>(x.y, m);
+                        x.z = dec_bf<uint32_t>(x.z, m);
+                        x.w = dec_bf<uint32_t>(x.w, m);
+                    }
                     *(uint4 *)&o[e] = x;
                 } else {
                     ulonglong2 x = *(const ulonglong2 *)&sk[lsw<K>(e)];
-                    x.x = dec_key((uint64_t)x.x, cout);
-                    x.y = dec_key((uint64_t)x.y, cout);
+                    if (cout) {
+                        x.x = dec_bf<uint64_t>((uint64_t)x.x, m);
+                        x.y = dec_bf<uint64_t>((uint64_t)x.y, m);
+                    }
                     *(ulonglong2 *)&o[e] = x;
                 }
             }

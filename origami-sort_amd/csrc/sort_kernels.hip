@@ -31,9 +31,29 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     K r[IPT];
     uint64_t v[IPT];
     int nmax = 0;
+    bool loaded = false;
+    if constexpr (!HV) {
+        if (cnt == TILE) {
+            // full key-only tile: loads off one base pointer, codec without
+            // per-key branches
+            const K *ip = (const K *)in + base;
+#pragma unroll
+            for (int i = 0; i < IPT; i++) {
+                r[i] = ip[(uint32_t)(i * BLOCK + tid)];
+                v[i] = 0;
+            }
+            if (cin) {
+                const CodecMask<K> m = codec_mask<K>(cin);
+#pragma unroll
+                for (int i = 0; i < IPT; i++) r[i] = enc_bf(r[i], m);
+            }
+            loaded = true;
+        }
+    }
     // all IPT loads are issued before any is used (indices past the end are
     // clamped to the last item, then replaced by pads)
-    if (HV && vin) {
+    if (loaded) {
+    } else if (HV && vin) {
         // structure of arrays: keys from in[], values from vin[]
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
@@ -49,12 +69,14 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
             GMem<K, HV>::load(in, base + (e < cnt ? e : cnt - 1), r[i], v[i]);
         }
     }
+    if (!loaded) {
 #pragma unroll
-    for (int i = 0; i < IPT; i++) {
-        const bool live = i * BLOCK + tid < cnt;
-        r[i] = live ? enc_key(r[i], cin) : KMAX;
-        v[i] = live ? v[i] : 0;
-        if constexpr (HV) nmax += live && r[i] == KMAX;
+        for (int i = 0; i < IPT; i++) {
+            const bool live = i * BLOCK + tid < cnt;
+            r[i] = live ? enc_key(r[i], cin) : KMAX;
+            v[i] = live ? v[i] : 0;
+            if constexpr (HV) nmax += live && r[i] == KMAX;
+        }
     }
     int lim = cnt;
     if constexpr (HV) {

@@ -54,11 +54,16 @@ def split(X, v, up, slog2, samples_j):
     return a
 
 
-def corank_rev(L, a_off, la, b_top, lb, diag):
-    """lds_corank_lift_rev: A items among the first diag outputs (ties to A)."""
+def pow2_floor(x):
+    return 1 << (x.bit_length() - 1) if x > 0 else 0
+
+
+def corank_rev(L, a_off, la, b_top, lb, diag, top):
+    """lds_corank_lift_rev: A items among the first diag outputs (ties to A);
+    top is the level's largest step bound over all pairs (any top with
+    2 * top > min(la, lb) gives the same answer)."""
     lo = max(0, diag - lb)
     hi = min(diag, la)
-    top = 1 << (min(la, lb).bit_length() - 1) if min(la, lb) > 0 else 0
     a = lo
     s = top
     while s > 0:
@@ -132,6 +137,9 @@ def model_tile(runs, lo, hi, block, ipt):
     logkw = kw.bit_length() - 1
     for lev in range(1, logkw + 1):
         regs = {}
+        hv = 1 << (lev - 1)
+        top = pow2_floor(max(min(P[s0 + hv] - P[s0], P[s0 + 2 * hv] - P[s0 + hv])
+                             for s0 in range(0, kw, 2 * hv)))
         for tid in range(block):
             c0 = tid * ipt
             if c0 >= Np:
@@ -141,7 +149,7 @@ def model_tile(runs, lo, hi, block, ipt):
             s0 = (j0 >> lev) << lev
             a_off, mid, end = P[s0], P[s0 + hv], P[s0 + 2 * hv]
             la, lb, diag = mid - a_off, end - mid, c0 - a_off
-            a = corank_rev(L, a_off, la, end - 1, lb, diag)
+            a = corank_rev(L, a_off, la, end - 1, lb, diag, top)
             regs[tid] = (merge_rev(L, a_off + a, end - 1 - (diag - a), ipt), j0, end, diag)
         for tid, (r, j0, end, diag) in regs.items():
             c0 = tid * ipt
