@@ -126,6 +126,20 @@ template <typename T> __device__ __forceinline__ int lsw(int i) {
 // items per 16-byte chunk
 template <typename T> constexpr int chunk_items() { return 16 / (int)sizeof(T); }
 
+// The swizzle on byte offsets (for 4- and 8-byte items alike: the 16-byte
+// chunk index within a 128-byte row XOR the row number): lets merge loops keep
+// their pointers in bytes and address LDS with two instructions.
+__device__ __forceinline__ int lsw_bytes(int b) {
+#if OGS_LDS_SWIZZLE
+    return b ^ ((b >> 3) & 0x70);
+#else
+    return b;
+#endif
+}
+template <typename K> __device__ __forceinline__ K lds_at_bytes(const K *sk, int b) {
+    return *(const K *)((const char *)sk + lsw_bytes(b));
+}
+
 // ----------------------------------------------------------------------------
 // Compare-exchange (SWAP / SWAPKV2, commons.h:111-149): a <- smaller, b <- larger.
 // ----------------------------------------------------------------------------
@@ -390,11 +404,13 @@ __device__ __forceinline__ int lds_corank_lift_rev(const K *sk, int a_off, int l
                                                    int top) {
     const int lo = diag > lb ? diag - lb : 0;
     const int hi = diag < la ? diag : la;
+    constexpr int W = (int)sizeof(K);
+    const int pa = (a_off - 1) * W, pb = (b_top - diag) * W;  // byte bases
     int a = lo;
     for (int s = top; s > 0; s >>= 1) {
         const int t = min(a + s, hi);
-        const K ka = sk[lsw<K>(a_off + t - 1)];
-        const K kb = sk[lsw<K>(b_top - diag + t)];
+        const K ka = lds_at_bytes(sk, pa + t * W);
+        const K kb = lds_at_bytes(sk, pb + t * W);
         a = kb < ka ? a : t;
     }
     return a;
@@ -409,16 +425,19 @@ __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i
         // One pointer: after i + 1 outputs the B pointer is ib0 - (i + 1 - (ia -
         // ia0)) = ia + dlt - (i + 1), with dlt = ib0 - ia0, so each step updates
         // ia alone and picks the next LDS index as ia or ia + dlt - (i + 1).
-        const int dlt = ib - ia;
+        // pointers in bytes (W per item)
+        constexpr int W = (int)sizeof(K);
+        const int dlt = (ib - ia) * W;
+        int pa = ia * W;
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
             const bool takeA = !(kb < ka);
             r[i] = takeA ? ka : kb;
-            const int ia1 = ia + 1;
-            const int nb = ia + dlt - (i + 1);
-            const int nx = takeA ? ia1 : nb;
-            ia = takeA ? ia1 : ia;
-            const K nk = sk[lsw<K>(nx)];
+            const int pa1 = pa + W;
+            const int nb = pa + dlt - W * (i + 1);
+            const int nx = takeA ? pa1 : nb;
+            pa = takeA ? pa1 : pa;
+            const K nk = lds_at_bytes(sk, nx);
             ka = takeA ? nk : ka;
             kb = takeA ? kb : nk;
         }
