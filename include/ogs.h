@@ -190,8 +190,13 @@ int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n
  * a plain sort instead; 0 = never), "partition_lanes" (lanes per merge tile
  * in the partition kernel: 1, 4, 8 or 16), "plain_fanin" (2: one merge level per
  * HBM pass; 4: two levels per pass, the last pass 2-way when one level is
- * left), "quad_lanes" (lanes per tile split in a 4-way pass: 1, 2, 4 or 8).  Block and merge parameters take an index
- * into ogs_shape_info's lists; *_max take an item count.  Segments of up to
+ * left), "quad_lanes" (lanes per tile split in a 4-way pass: 1, 2, 4 or 8),
+ * "kway_fanin" (key-only kinds: the largest fan-in of the sample-split K-way
+ * merge pass, 2, 4, 8, 16 or 32, which replaces the 2- and 4-way passes of a
+ * plain sort; 0 = off), "kway_shape" (index into ogs_shape_info's K-way
+ * list), "kway_sample" (log2 of its sample spacing, 3-10).  Block, merge and
+ * K-way shape parameters take an index into ogs_shape_info's lists; *_max
+ * take an item count.  Segments of up to
  * seg_tiny_max items (0 = no tiny class) get one workgroup of the tiny shape
  * each, segments up to seg_small_max (0 = the small shape's tile) one
  * workgroup of the small shape, longer ones are cut into tiles of the large
@@ -207,13 +212,16 @@ int ogs_tune_load(ogs_ctx *ctx, const char *path);
 
 /* Compiled kernel shapes of a key type: merge = 0 lists block-sort shapes
  * (workgroup size, items per lane, in-wave bitonic levels), merge = 1 merge
- * shapes (wlev = -1).  Fills entry idx when it exists; returns the count. */
+ * shapes, merge = 2 K-way merge shapes (both wlev = -1; no K-way shapes for
+ * key-value kinds).  Fills entry idx when it exists; returns the count. */
 int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *wlev);
 
 /* Per-kernel timing with HIP events recorded on the launching stream, for
  * plain sorts issued while enabled.  kind: 0 block sort, 1 merge partition
- * (co-rank tables), 2 2-way merge pass, 3 4-way merge pass.  read returns the summed milliseconds, the launch count and
- * the items those launches covered (waits for the recorded events). */
+ * (co-rank tables; for K-way passes the sample merges and the tile split
+ * table), 2 2-way merge pass, 3 4-way merge pass, 4 K-way merge pass.  read
+ * returns the summed milliseconds, the launch count and the items those
+ * launches covered (waits for the recorded events). */
 int ogs_profile_enable(ogs_ctx *ctx, int on);
 int ogs_profile_read(ogs_ctx *ctx, int kind, double *total_ms, uint64_t *launches, uint64_t *items);
 int ogs_profile_reset(ogs_ctx *ctx);

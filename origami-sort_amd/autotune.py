@@ -118,6 +118,42 @@ def tune_class(ctx, name, kind, quick):
                 result["quad_lanes"] = lanes
         result.setdefault("quad_lanes", qlanes0)
         ctx.tune_set(kind, "quad_lanes", result["quad_lanes"])
+    result["kway_fanin"] = 0
+    if w <= 8:
+        # key-only kinds: sample-split K-way passes (fan-in x shape x sample
+        # spacing x block-sort shape); kept only when faster than the best
+        # 2- and 4-way plan above
+        ks = ogs.shapes(kind, 2)
+        kbest = None
+        for bi, (bb, bp, bw) in enumerate(bs):
+            if bb * bp < 4096 or bb * bp & (bb * bp - 1):
+                continue
+            for ki in range(len(ks)):
+                for fanin in (8, 16, 32):
+                    for sample in (5, 6, 7):
+                        ctx.tune_set(kind, "plain_block", bi)
+                        ctx.tune_set(kind, "kway_fanin", fanin)
+                        ctx.tune_set(kind, "kway_shape", ki)
+                        ctx.tune_set(kind, "kway_sample", sample)
+                        try:
+                            t = timed(lambda: ctx.sort_keep_ptr(kind, d_in.data_ptr(), a.data_ptr(), b.data_ptr(), n),
+                                      reps)
+                        except ogs.OgsError as e:  # a shape that cannot hold the slices
+                            log(f"  {name} kway block={bs[bi]} shape={ks[ki]} fanin={fanin}: {e}")
+                            continue
+                        log(f"  {name} kway block={bs[bi]} shape={ks[ki]} fanin={fanin} sample={sample}: "
+                            f"{t * 1e3:.3f} ms")
+                        if kbest is None or t < kbest[0]:
+                            kbest = (t, bi, ki, fanin, sample)
+        if kbest is not None and kbest[0] < best[0]:
+            best = (kbest[0],) + best[1:]
+            result["plain_block"] = kbest[1]
+            result["kway_shape"], result["kway_fanin"], result["kway_sample"] = kbest[2], kbest[3], kbest[4]
+        ctx.tune_set(kind, "plain_block", result["plain_block"])
+        ctx.tune_set(kind, "kway_fanin", result["kway_fanin"])
+        if result["kway_fanin"]:
+            ctx.tune_set(kind, "kway_shape", result["kway_shape"])
+            ctx.tune_set(kind, "kway_sample", result["kway_sample"])
     result["_plain"] = {"n": n, "ms": best[0] * 1e3, "gkeys": n / best[0] / 1e9}
     del d_in, a, b
     # ---- segmented batch: log-uniform lengths in [1, 2^20]

@@ -1,6 +1,6 @@
 // kway_kernels.hip -- the sample-split K-way merge pass (key-only classes).
 //
-// One HBM pass merges groups of KW sorted runs (KW = 2, 4, 8, 16): log2(KW)
+// One HBM pass merges groups of KW sorted runs (KW = 2 ... 32): log2(KW)
 // levels of the reference's merge tree (MergeTreeEven / Merge4Way,
 // merge_tree.h:319-3010) per read and write of the data, with the tiles cut by
 // the role of Partition_Ptrs (Partition.h:117-166) for KW runs at once.
@@ -142,10 +142,10 @@ template <typename K, int KW, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, K *__restrict__ out, uint64_t n,
                                                       int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
                                                       const uint64_t *__restrict__ tab, SampleSpec sp) {
-    static_assert(KW == 2 || KW == 4 || KW == 8 || KW == 16, "fan-in 2, 4, 8 or 16");
+    static_assert(KW == 2 || KW == 4 || KW == 8 || KW == 16 || KW == 32, "fan-in 2, 4, 8, 16 or 32");
     static_assert((IPT & (IPT - 1)) == 0 && IPT >= 4, "IPT: a power of two >= 4");
     constexpr int C = BLOCK * IPT;
-    constexpr int LOGKW = KW == 2 ? 1 : KW == 4 ? 2 : KW == 8 ? 3 : 4;
+    constexpr int LOGKW = KW == 2 ? 1 : KW == 4 ? 2 : KW == 8 ? 3 : KW == 16 ? 4 : 5;
     constexpr K KMAX = KeyMax<K>::v;
     __shared__ __attribute__((aligned(16))) K skb[C + lds_extra<K, IPT>()];
     __shared__ uint64_t s_src[KW];  // global index of each slice's first item
@@ -327,7 +327,7 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     constexpr uint64_t C = (uint64_t)BLOCK * IPT;
     if (n == 0) return hipSuccess;
     const uint64_t Q = kw_q(C, IPT, KW, slog2);
-    if (Q == 0 || rlog2 - slog2 > 28 || rlog2 < slog2) return hipErrorInvalidValue;
+    if (Q == 0 || rlog2 - slog2 > 27 || rlog2 < slog2) return hipErrorInvalidValue;
     const uint64_t GR = (1ull << rlog2) * KW;
     const uint64_t ntg = kw_group_tiles(GR >> slog2, Q);
     const uint64_t nt_last = kw_group_tiles(kw_last_samples(n, rlog2, slog2, KW), Q);
@@ -342,10 +342,11 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     return hipGetLastError();
 }
 
-#define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16)
-#define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8)
-#define OGS_KX(K, B, I) \
-    {B, I, {&launch_kway<K, 2, B, I>, &launch_kway<K, 4, B, I>, &launch_kway<K, 8, B, I>, &launch_kway<K, 16, B, I>}},
+#define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16) X(1024, 32)
+#define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8) X(1024, 16)
+#define OGS_KX(K, B, I)                                                                                       \
+    {B, I, {&launch_kway<K, 2, B, I>, &launch_kway<K, 4, B, I>, &launch_kway<K, 8, B, I>, &launch_kway<K, 16, B, I>, \
+            &launch_kway<K, 32, B, I>}},
 
 static const KwayShapeFns kKway32[] = {
 #define X(B, I) OGS_KX(uint32_t, B, I)

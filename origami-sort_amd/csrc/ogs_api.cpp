@@ -333,7 +333,7 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
     int rl = ilog2(T), smin = 63;
     for (int p = 0; p < P; p++) {
         sl[p] = std::min(tune[ki.kc].v[TP_KWAY_SAMPLE], rl - 8);
-        if (sl[p] < 0 || rl - sl[p] > 28)
+        if (sl[p] < 0 || rl - sl[p] > 27)
             return fail(OGS_ERR_ARG, "K-way pass %d: no sample spacing for runs of 2^%d", p, rl);
         smin = std::min(smin, sl[p]);
         rl += lv[p];

@@ -226,8 +226,8 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
         return OGS_OK;
     }
     else if (p == TP_KWAY_FANIN) {
-        if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16)
-            return fail(OGS_ERR_ARG, "kway_fanin must be 0, 2, 4, 8 or 16 (got %d)", value);
+        if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
+            return fail(OGS_ERR_ARG, "kway_fanin must be 0, 2, 4, 8, 16 or 32 (got %d)", value);
         reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
         return OGS_OK;
     }
@@ -296,7 +296,14 @@ int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *
     KeyInfo ki;
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
     int cnt = 0;
-    if (merge) {
+    if (merge == 2) {
+        const KwayShapeFns *t = kway_shapes(ki.kc, &cnt);
+        if (idx >= 0 && idx < cnt) {
+            if (block) *block = t[idx].block;
+            if (ipt) *ipt = t[idx].ipt;
+            if (wlev) *wlev = -1;
+        }
+    } else if (merge) {
         const MergeShapeFns *t = merge_shapes(ki.kc, &cnt);
         if (idx >= 0 && idx < cnt) {
             if (block) *block = t[idx].block;

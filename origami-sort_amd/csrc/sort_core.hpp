@@ -598,8 +598,9 @@ __device__ __forceinline__ void lds_store_tile(const K *sk, const uint64_t *sv, 
 // tile are output items g0 .. g0+cnt-1; every item whose output index is a
 // multiple of 2^slog2 is written to smp[index >> slog2] with its run r within
 // the next pass's group of kw runs of 2^rlog2 items and its index m in that
-// run over 2^slog2: 32-bit keys as u64 key << 32 | r << 28 | m (u64 order is
-// then the total order (key, run, index)), 64-bit keys as {key, r << 32 | m}.
+// run over 2^slog2: 32-bit keys as u64 key << 32 | r << 27 | m (r < 32, m <
+// 2^27; u64 order is then the total order (key, run, index)), 64-bit keys as
+// {key, r << 32 | m}.
 // ----------------------------------------------------------------------------
 template <int BLOCK, typename K>
 __device__ __forceinline__ void write_samples(const K *sk, int cnt, uint64_t g0, void *smp, int rlog2, int kw,
@@ -613,7 +614,7 @@ __device__ __forceinline__ void write_samples(const K *sk, int cnt, uint64_t g0,
         const uint64_t m = (gi - (run << rlog2)) >> slog2;
         const K key = sk[lsw<K>(e)];
         if constexpr (sizeof(K) == 4)
-            ((uint64_t *)smp)[gi >> slog2] = ((uint64_t)key << 32) | (r << 28) | m;
+            ((uint64_t *)smp)[gi >> slog2] = ((uint64_t)key << 32) | (r << 27) | m;
         else
             ((ulonglong2 *)smp)[gi >> slog2] = make_ulonglong2((unsigned long long)key, (r << 32) | m);
     }
@@ -628,8 +629,8 @@ template <> struct Sample<uint32_t> {
     static __device__ __forceinline__ void get(const void *p, uint64_t i, uint32_t &k, int &r, uint64_t &m) {
         const uint64_t x = ((const uint64_t *)p)[i];
         k = (uint32_t)(x >> 32);
-        r = (int)((x >> 28) & 15);
-        m = x & ((1ull << 28) - 1);
+        r = (int)((x >> 27) & 31);
+        m = x & ((1ull << 27) - 1);
     }
 };
 template <> struct Sample<uint64_t> {

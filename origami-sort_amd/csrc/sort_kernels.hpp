@@ -128,7 +128,7 @@ typedef hipError_t (*KwayLaunch)(const void *in, void *out, uint64_t n, int rlog
                                  hipEvent_t mid);
 struct KwayShapeFns {
     int block, ipt;
-    KwayLaunch run[4];  // fan-in 2, 4, 8, 16
+    KwayLaunch run[5];  // fan-in 2, 4, 8, 16, 32
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
 // Tiles of one K-way pass (0 when the shape cannot hold kw slices at this

@@ -63,3 +63,27 @@ def test_shape_registry():
         for b, ipt, w in ms:
             tile = b * ipt
             assert tile & (tile - 1) == 0 and w == -1
+
+
+def test_kway_shape_registry():
+    # K-way merge shapes exist for key-only classes only; every block is whole
+    # LDS swizzle blocks (256 items for 4-byte keys, 128 for 8-byte)
+    for kind, blk in ((ogs.U32, 256), (ogs.U64, 128)):
+        ks = ogs.shapes(kind, 2)
+        assert ks
+        for b, ipt, w in ks:
+            assert b % blk == 0 and ipt & (ipt - 1) == 0 and w == -1
+    assert ogs.shapes(ogs.KV_U64U64, 2) == []
+
+
+def test_kway_params_checked():
+    try:
+        ctx = ogs.Context(0)
+    except ogs.OgsError:
+        pytest.skip("creating a context needs a device")
+    for bad in (("kway_fanin", 3), ("kway_fanin", 64), ("kway_sample", 2), ("kway_sample", 11)):
+        with pytest.raises(ogs.OgsError):
+            ctx.tune_set(ogs.U32, *bad)
+    ctx.tune_set(ogs.U32, "kway_fanin", 32)
+    assert ctx.tune_get(ogs.U32, "kway_fanin") == 32
+    ctx.close()

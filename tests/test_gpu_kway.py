@@ -1,5 +1,5 @@
 """GPU parity of the sample-split K-way merge pass (kway_kernels.hip) against the
-oracle: key-only kinds, every fan-in (2, 4, 8, 16), sample spacings, tile shapes,
+oracle: key-only kinds, every fan-in (2, 4, 8, 16, 32), sample spacings, tile shapes,
 distributions with heavy ties, ragged sizes, and a large size checked against an
 independent device sort of the same keys."""
 import sys
@@ -58,7 +58,7 @@ def check(kind, got, a):
 
 
 @pytest.mark.parametrize("kind", KEY_ONLY)
-@pytest.mark.parametrize("fanin", [2, 4, 8, 16])
+@pytest.mark.parametrize("fanin", [2, 4, 8, 16, 32])
 @pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.ZIPF, ogs.SORTED, ogs.REVERSE, ogs.DISTINCT16])
 def test_kway_kinds(ctx, kind, fanin, dist):
     n = 300007
@@ -72,7 +72,7 @@ def test_kway_kinds(ctx, kind, fanin, dist):
 
 
 @pytest.mark.parametrize("n", [2049, 4096, 4097, 16385, 65536, 65537, 262143, 1000003])
-@pytest.mark.parametrize("fanin", [4, 16])
+@pytest.mark.parametrize("fanin", [4, 16, 32])
 def test_kway_sizes(ctx, n, fanin):
     say(f"case kway u32 sizes fanin={fanin} n={n}")
     configure(ctx, ogs.U32, fanin)
@@ -85,7 +85,7 @@ def test_kway_sizes(ctx, n, fanin):
 
 @pytest.mark.parametrize("kind", [ogs.U32, ogs.U64])
 @pytest.mark.parametrize("sample", [3, 4, 6, 7])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 def test_kway_shapes_and_spacing(ctx, kind, sample, shape):
     n = 777777
     say(f"case kway kind={ogs.KIND_NAMES[kind]} sample={sample} shape={shape} n={n}")

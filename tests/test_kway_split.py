@@ -223,12 +223,13 @@ def runs_of(keys, R):
     return k
 
 
-@pytest.mark.parametrize("kw", [2, 4, 8, 16])
+@pytest.mark.parametrize("kw", [2, 4, 8, 16, 32])
 @pytest.mark.parametrize("dist", ["uniform", "few", "sorted", "reverse"])
 def test_kway_pass_model(kw, dist):
     rng = np.random.default_rng(kw * 7 + len(dist))
-    block, ipt, slog2 = 64, 8, 3
-    R = 1024
+    # 32 padded slices need a larger tile; shorter runs keep the model quick
+    block, ipt, slog2 = (64, 8, 3) if kw < 32 else (128, 8, 3)
+    R = 1024 if kw < 32 else 256
     n = R * kw * 2 + 3 * R + 77  # two full groups and a ragged one
     if dist == "uniform":
         keys = rng.integers(0, 2**32, n, dtype=np.uint64)
@@ -258,8 +259,8 @@ def test_kway_pass_model_shapes(kw, ipt, slog2):
 
 def test_kway_tile_bounds_hold_at_headline_shapes():
     """Q leaves room for the padded slices at the shapes the library registers."""
-    for C, ipt in ((16384, 32), (8192, 32), (8192, 16), (4096, 16), (4096, 8), (2048, 8)):
-        for kw in (2, 4, 8, 16):
+    for C, ipt in ((32768, 32), (16384, 32), (16384, 16), (8192, 32), (8192, 16), (4096, 16), (4096, 8), (2048, 8)):
+        for kw in (2, 4, 8, 16, 32):
             for slog2 in range(3, 8):
                 q = kw_q(C, ipt, kw, slog2)
                 if q == 0:
