@@ -47,6 +47,28 @@ def traffic_from_profiles(kernel, workload):
         return None
 
 
+def valu_from_profiles(kernel, workload, n):
+    """The dominant kernel's VALU issue figures from the committed rocprofv3 SQ
+    counters (profiles/pmc_traffic.json), when they are of this workload: its
+    instructions per key and its issue rate against the one-instruction-per-CU-
+    cycle ceiling of a wave64 VALU op on four 16-lane SIMDs."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        if d.get("bench_workload") != workload:
+            return None
+        v = d.get(kernel, {}).get("valu")
+        if not v:
+            return None
+        # SQ_INSTS_VALU counts wave64 instructions: 64 lane operations each
+        return {"insts_per_key": round(v["insts_per_launch"] * 64 / n, 1),
+                "per_cu_cycle": v["per_cu_cycle"], "ceiling_per_cu_cycle": 1.0,
+                "source": "profiles/pmc_traffic.json (rocprofv3 SQ_INSTS_VALU, SQ_BUSY_CYCLES)"}
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def cpu_baseline(kind_oracle, host_keys, sample_n, threads):
     import numpy as np
 
@@ -433,6 +455,10 @@ def main():
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
+    valu = valu_from_profiles(dom, workload, n)
+    if valu:
+        # the dominant kernel's issue bound (it runs below the HBM roofline)
+        line["valu"] = dict(kernel=dom, **valu)
     if args.copy_calib:
         for _ in range(3):
             d_b.copy_(d_in)
