@@ -1,4 +1,4 @@
-// kway_kernels.hip -- the sample-split K-way merge pass (key-only classes).
+// kway_kernels.hip -- the sample-split K-way merge pass.
 //
 // One HBM pass merges groups of KW sorted runs (KW = 2 ... 32): log2(KW)
 // levels of the reference's merge tree (MergeTreeEven / Merge4Way,
@@ -22,12 +22,16 @@
 //
 // Merge.  The workgroup loads the KW slices into LDS, each padded with maximum
 // keys to a multiple of IPT (so every pair of runs at every level starts on a
-// thread boundary and no thread's IPT outputs straddle two pairs), the B side
-// of every pair stored descending (the bitonic layout of the block sort's
-// rounds, so merges test no exhaustion), merges the pairs level by level
-// through registers, and writes the first cnt items (the real ones: pads are
-// maximum keys and sort last; for key-only data equal keys are identical bits)
-// to the output at the group base plus the sum of the tile's start splits.
+// thread boundary and no thread's IPT outputs straddle two pairs), merges the
+// pairs level by level through registers, and writes the first cnt items (the
+// real ones) to the output at the group base plus the sum of the tile's start
+// splits.  Key-only: the B side of every pair is stored descending (the
+// bitonic layout of the block sort's rounds, so merges test no exhaustion);
+// pads are maximum keys and sort last, and equal keys are identical bits.
+// Key-value (16-byte {key, value} records, keys and values in two LDS arrays):
+// every run ascending, and merges test exhaustion against each run's real
+// length, so the merged pair is its real items then its pads and a pad never
+// displaces a real maximum-key item.
 #include "sort_core.hpp"
 #include "sort_kernels.hpp"
 
@@ -74,8 +78,8 @@ uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int
 // run j's samples (which number precede the boundary), then over the s items
 // that separate two of them.
 // ----------------------------------------------------------------------------
-template <typename K, int KW>
-__global__ void k_kway_bounds(const K *__restrict__ in, uint64_t n, int rlog2, int slog2, uint64_t Q, uint64_t ntg,
+template <typename K, bool HV, int KW>
+__global__ void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2, int slog2, uint64_t Q, uint64_t ntg,
                               uint64_t ntiles, const void *__restrict__ s0, const void *__restrict__ sm,
                               uint64_t *__restrict__ tab) {
     const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,13 +111,13 @@ __global__ void k_kway_bounds(const K *__restrict__ in, uint64_t n, int rlog2, i
             }
             if (lo > 0) {
                 // sample lo-1 precedes b and sample lo (if any) does not
-                const K *X = in + gbase + (uint64_t)j * R;
+                const uint64_t X = gbase + (uint64_t)j * R;
                 uint64_t a = ((lo - 1) << slog2) + 1;
                 uint64_t b = lo << slog2;
                 b = b < lenj ? b : lenj;
                 while (a < b) {
                     const uint64_t mid = (a + b) >> 1;
-                    const K k = X[mid];
+                    const K k = GMem<K, HV>::load_key(in, X + mid);
                     if (up ? !(v < k) : k < v) a = mid + 1;
                     else b = mid;
                 }
@@ -138,8 +142,8 @@ __device__ __forceinline__ void store_items(const K *sp0, K *ob, int tid) {
 // ----------------------------------------------------------------------------
 // The merge: one workgroup per tile.
 // ----------------------------------------------------------------------------
-template <typename K, int KW, int BLOCK, int IPT>
-__global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, K *__restrict__ out, uint64_t n,
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
                                                       int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
                                                       const uint64_t *__restrict__ tab, SampleSpec sp) {
     static_assert(KW == 2 || KW == 4 || KW == 8 || KW == 16 || KW == 32, "fan-in 2, 4, 8, 16 or 32");
@@ -148,13 +152,16 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
     constexpr int LOGKW = KW == 2 ? 1 : KW == 4 ? 2 : KW == 8 ? 3 : KW == 16 ? 4 : 5;
     constexpr K KMAX = KeyMax<K>::v;
     __shared__ __attribute__((aligned(16))) K skb[C + lds_extra<K, IPT>()];
+    __shared__ __attribute__((aligned(16))) uint64_t svb[HV ? C + lds_extra<K, IPT>() : 2];  // values (KV)
     __shared__ uint64_t s_src[KW];  // global index of each slice's first item
     __shared__ int s_n[KW];         // items per slice
     __shared__ int s_P[KW + 1];     // prefix sums of the padded slice lengths
+    __shared__ int s_R[KW + 1];     // prefix sums of the real slice lengths
     __shared__ uint64_t s_o0;       // output index of the tile's first item
     __shared__ int s_cnt;           // items in the tile (-1: inconsistent table)
     __shared__ int s_top[LOGKW + 1];  // per level: co-rank step bound over all pairs
     K *sk = skb + lds_front<K>();
+    uint64_t *sv = svb + (HV ? lds_front<K>() : 0);
     const int tid = threadIdx.x, lane = tid & 63;
     const uint64_t t = blockIdx.x;
     const uint64_t R = 1ull << rlog2;
@@ -175,24 +182,28 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
         bool bad = false;
         uint64_t o0 = gbase;
         s_P[0] = 0;
+        s_R[0] = 0;
         for (int j = 0; j < KW; j++) {
             const int nj = s_n[j];
             bad |= nj < 0;
             cnt += nj;
             p += (nj + IPT - 1) & ~(IPT - 1);
             s_P[j + 1] = p;
+            s_R[j + 1] = cnt;
             o0 += s_src[j] - (gbase + (uint64_t)j * R);
         }
         s_o0 = o0;
         s_cnt = bad || p > C ? -1 : cnt;  // cannot happen with a consistent table
         // the co-rank of a pair (la, lb) needs pow2_floor(min(la, lb)) lifting
-        // steps; every thread of a level takes the largest, so the step loop
-        // is uniform (more steps than a pair needs are harmless)
+        // steps (KV: the real lengths); every thread of a level takes the
+        // largest, so the step loop is uniform (more steps than a pair needs
+        // are harmless)
         for (int lev = 1; lev <= LOGKW; lev++) {
             int mx = 0;
             for (int s0 = 0; s0 < KW; s0 += 2 << (lev - 1)) {
                 const int hv = 1 << (lev - 1);
-                const int la = s_P[s0 + hv] - s_P[s0], lb = s_P[s0 + 2 * hv] - s_P[s0 + hv];
+                const int *Q = HV ? s_R : s_P;
+                const int la = Q[s0 + hv] - Q[s0], lb = Q[s0 + 2 * hv] - Q[s0 + hv];
                 mx = max(mx, min(la, lb));
             }
             s_top[lev] = pow2_floor(mx);
@@ -211,14 +222,14 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
 #pragma unroll
     for (int b = KW / 2; b >= 1; b >>= 1) j0 += s_P[j0 + b] <= c0 ? b : 0;
     // chunk source: position c0 + r holds item base + r (ascending slices) or
-    // base - r (odd slices, the B side of the first level, stored descending);
-    // real items are r in [v0, v1), the rest are pads
+    // base - r (key-only odd slices, the B side of the first level, stored
+    // descending); real items are r in [v0, v1), the rest are pads
     uint32_t cb_lo = 0, cb_hi = 0, cmeta = 0;
     if (live) {
         const int pj = s_P[j0], npj = s_P[j0 + 1] - pj, nj = s_n[j0];
         const uint64_t src = s_src[j0];
         const int q = c0 - pj;
-        const bool desc = (j0 & 1) != 0;
+        const bool desc = !HV && (j0 & 1) != 0;
         int v0, v1;
         uint64_t base;
         if (desc) {
@@ -237,20 +248,20 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
         cmeta = (uint32_t)(v0 | (v1 << 8) | ((int)desc << 16));
     }
     {
-        // wave w loads LDS positions [w * 64 * IPT, (w + 1) * 64 * IPT): the
-        // chunks of its own lanes; load i covers positions 64 * i + lane of that
-        // range, in chunk i * (64 / IPT) + lane / IPT, whose source the lane
-        // fetches from the owning lane.  All loads are in flight at once; pads
-        // read the group's first item and keep the maximum key.
-        // Load i covers positions wbase + BLK * (i % NB) + 64 * (i / NB) + lane,
-        // BLK = the index span the LDS swizzle repeats over, so a lane's LDS
-        // index is its group's swizzled base plus a multiple of BLK.
+        // Wave w loads LDS positions [w * 64 * IPT, (w + 1) * 64 * IPT), the
+        // chunks of its own lanes: load i covers positions wbase + BLK * (i %
+        // NB) + 64 * (i / NB) + lane (BLK = the index span the LDS swizzle
+        // repeats over, so a lane's LDS index is its group's swizzled base plus
+        // a multiple of BLK), and the lane fetches its chunk's source from the
+        // owning lane.  All loads are in flight at once; pads read the group's
+        // first item and keep the maximum key.
         constexpr int BLK = sizeof(K) == 4 ? 256 : 128;
         constexpr int NB = 64 * IPT / BLK;
         static_assert(NB >= 1 && 64 * IPT % BLK == 0, "a wave's load range is whole swizzle blocks");
         const int r = lane & (IPT - 1);
         const int wbase = (tid >> 6) * 64 * IPT;
         K kk[IPT];
+        uint64_t vv[IPT];
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
             const int pw = BLK * (i % NB) + 64 * (i / NB) + lane;  // position within the wave's range
@@ -266,16 +277,22 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
             const uint64_t b = ((uint64_t)bhi << 32) | blo;
             const uint64_t mask = (uint64_t)0 - (uint64_t)real;
             const uint64_t src = gbase + ((b + (uint64_t)off - gbase) & mask);
-            const K x = in[src];
+            K x;
+            uint64_t xv = 0;
+            GMem<K, HV>::load(in, src, x, xv);
             kk[i] = real ? x : KMAX;
+            vv[i] = real ? xv : 0;
         }
 #pragma unroll
-        for (int i = 0; i < IPT; i++)
-            sk[lsw<K>(wbase + 64 * (i / NB) + lane) + BLK * (i % NB)] = kk[i];
+        for (int i = 0; i < IPT; i++) {
+            const int e = lsw<K>(wbase + 64 * (i / NB) + lane) + BLK * (i % NB);
+            sk[e] = kk[i];
+            if constexpr (HV) sv[e] = vv[i];
+        }
     }
     __syncthreads();
     K r[IPT];
-    uint64_t vdum[IPT];
+    uint64_t v[IPT];
 #pragma unroll 1
     for (int lev = 1; lev <= LOGKW; lev++) {
         int end = 0, diag = 0;
@@ -284,16 +301,35 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
             const int s0 = (j0 >> lev) << lev;
             const int a_off = s_P[s0], mid = s_P[s0 + hv];
             end = s_P[s0 + 2 * hv];
-            const int la = mid - a_off, lb = end - mid;
             diag = c0 - a_off;
             const int top = __builtin_amdgcn_readfirstlane(s_top[lev]);
-            lds_merge_slice_rev<IPT>(sk, a_off, la, end - 1, lb, diag, IPT, top, r);
+            if constexpr (!HV) {
+                lds_merge_slice_rev<IPT>(sk, a_off, mid - a_off, end - 1, end - mid, diag, IPT, top, r);
+            } else {
+                // KV: ascending runs; merges test exhaustion against the real
+                // lengths, so a pad never displaces a real maximum-key item and
+                // the merged pair is its real items, then pads
+                const int la = s_R[s0 + hv] - s_R[s0], lb = s_R[s0 + 2 * hv] - s_R[s0 + hv];
+                const int nreal = la + lb;
+                int c = nreal - diag;
+                c = c < 0 ? 0 : c > IPT ? IPT : c;
+                if (c > 0) {
+                    const int a = lds_corank_lift<K>(sk, a_off, la, mid, lb, diag, top);
+                    if (c == IPT) lds_merge_serial<IPT, true, true>(sk, sv, a_off, la, mid, lb, a, diag - a, 0, IPT, r, v);
+                    else lds_merge_serial<IPT, true, false>(sk, sv, a_off, la, mid, lb, a, diag - a, 0, c, r, v);
+                }
+#pragma unroll
+                for (int i = 0; i < IPT; i++) {
+                    r[i] = i < c ? r[i] : KMAX;
+                    v[i] = i < c ? v[i] : 0;
+                }
+            }
         }
         __syncthreads();
         if (live) {
-            // the next level's B sides are stored descending
-            if (lev < LOGKW && ((j0 >> lev) & 1)) lds_put_desc<IPT>(sk, end - 1 - diag, r);
-            else lds_put<IPT, false>(sk, nullptr, c0, r, vdum);
+            // the next level's B sides are stored descending (key-only)
+            if (!HV && lev < LOGKW && ((j0 >> lev) & 1)) lds_put_desc<IPT>(sk, end - 1 - diag, r);
+            else lds_put<IPT, HV>(sk, sv, c0, r, v);
         }
         __syncthreads();
     }
@@ -304,24 +340,39 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const K *__restrict__ in, 
     // it as their base and a 32-bit lane offset
     const uint64_t o0u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(o0 >> 32)) << 32) |
                          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o0);
-    K *ob = out + o0u;
     const K *sp0 = sk + lsw<K>(tid);
-    if (tid + (IPT - 1) * BLOCK < cnt) {
-        // all IPT items of this thread are real: every read, then every store
-        // (the codec is uniform: one branch picks the loop)
-        if (cout == 0) store_items<BLOCK, IPT, 0>(sp0, ob, tid);
-        else if (cout == 1) store_items<BLOCK, IPT, 1>(sp0, ob, tid);
-        else store_items<BLOCK, IPT, 2>(sp0, ob, tid);
-    } else {
-        const int nk = cnt > tid ? (cnt - tid + BLOCK - 1) / BLOCK : 0;
+    if constexpr (HV) {
+        ulonglong2 *ob = (ulonglong2 *)out + o0u;
+        const uint64_t *vp0 = sv + lsw<uint64_t>(tid);
+        const CodecMask<K> m = codec_mask<K>(cout);
 #pragma unroll
-        for (int k = 0; k < IPT; k++)
-            if (k < nk) ob[(uint32_t)(tid + k * BLOCK)] = dec_key(sp0[k * BLOCK], cout);
+        for (int k = 0; k < IPT; k++) {
+            if (tid + k * BLOCK < cnt) {
+                ulonglong2 q;
+                q.x = (unsigned long long)dec_bf(sp0[k * BLOCK], m);
+                q.y = (unsigned long long)vp0[k * BLOCK];
+                ob[(uint32_t)(tid + k * BLOCK)] = q;
+            }
+        }
+    } else {
+        K *ob = (K *)out + o0u;
+        if (tid + (IPT - 1) * BLOCK < cnt) {
+            // all IPT items of this thread are real: every read, then every
+            // store (the codec is uniform: one branch picks the loop)
+            if (cout == 0) store_items<BLOCK, IPT, 0>(sp0, ob, tid);
+            else if (cout == 1) store_items<BLOCK, IPT, 1>(sp0, ob, tid);
+            else store_items<BLOCK, IPT, 2>(sp0, ob, tid);
+        } else {
+            const int nk = cnt > tid ? (cnt - tid + BLOCK - 1) / BLOCK : 0;
+#pragma unroll
+            for (int k = 0; k < IPT; k++)
+                if (k < nk) ob[(uint32_t)(tid + k * BLOCK)] = dec_key(sp0[k * BLOCK], cout);
+        }
     }
     if (sp.smp) write_samples<BLOCK>(sk, cnt, o0, sp.smp, sp.rlog2, sp.kw, sp.slog2);
 }
 
-template <typename K, int KW, int BLOCK, int IPT>
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
 static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, int slog2, int cout, uint64_t *tab,
                               const void *s0, const void *sm, const SampleSpec *next, hipStream_t s, hipEvent_t mid) {
     constexpr uint64_t C = (uint64_t)BLOCK * IPT;
@@ -333,29 +384,35 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     const uint64_t nt_last = kw_group_tiles(kw_last_samples(n, rlog2, slog2, KW), Q);
     const uint64_t ntiles = (n + GR - 1) / GR * ntg - ntg + nt_last;
     const uint64_t threads = ntiles * KW;
-    hipLaunchKernelGGL((k_kway_bounds<K, KW>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, (const K *)in,
-                       n, rlog2, slog2, Q, ntg, ntiles, s0, sm, tab);
+    hipLaunchKernelGGL((k_kway_bounds<K, HV, KW>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, in, n,
+                       rlog2, slog2, Q, ntg, ntiles, s0, sm, tab);
     if (mid) (void)hipEventRecord(mid, s);
     const SampleSpec none = {nullptr, 0, 0, 0};
-    hipLaunchKernelGGL((k_kway_merge<K, KW, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, (const K *)in,
-                       (K *)out, n, rlog2, ntg, nt_last, cout, (const uint64_t *)tab, next ? *next : none);
+    hipLaunchKernelGGL((k_kway_merge<K, HV, KW, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n,
+                       rlog2, ntg, nt_last, cout, (const uint64_t *)tab, next ? *next : none);
     return hipGetLastError();
 }
 
 #define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16) X(1024, 32)
 #define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8) X(1024, 16)
-#define OGS_KX(K, B, I)                                                                                       \
-    {B, I, {&launch_kway<K, 2, B, I>, &launch_kway<K, 4, B, I>, &launch_kway<K, 8, B, I>, &launch_kway<K, 16, B, I>, \
-            &launch_kway<K, 32, B, I>}},
+#define OGS_KWAY_SHAPES_KV(X) X(512, 8) X(256, 16) X(256, 8) X(512, 4)
+#define OGS_KX(K, HV, B, I)                                                                                 \
+    {B, I, {&launch_kway<K, HV, 2, B, I>, &launch_kway<K, HV, 4, B, I>, &launch_kway<K, HV, 8, B, I>,       \
+            &launch_kway<K, HV, 16, B, I>, &launch_kway<K, HV, 32, B, I>}},
 
 static const KwayShapeFns kKway32[] = {
-#define X(B, I) OGS_KX(uint32_t, B, I)
+#define X(B, I) OGS_KX(uint32_t, false, B, I)
     OGS_KWAY_SHAPES_32(X)
 #undef X
 };
 static const KwayShapeFns kKway64[] = {
-#define X(B, I) OGS_KX(uint64_t, B, I)
+#define X(B, I) OGS_KX(uint64_t, false, B, I)
     OGS_KWAY_SHAPES_64(X)
+#undef X
+};
+static const KwayShapeFns kKwayKV[] = {
+#define X(B, I) OGS_KX(uint64_t, true, B, I)
+    OGS_KWAY_SHAPES_KV(X)
 #undef X
 };
 
@@ -363,7 +420,7 @@ const KwayShapeFns *kway_shapes(KeyClass kc, int *count) {
     switch (kc) {
     case KC_32: *count = (int)(sizeof(kKway32) / sizeof(kKway32[0])); return kKway32;
     case KC_64: *count = (int)(sizeof(kKway64) / sizeof(kKway64[0])); return kKway64;
-    default: *count = 0; return nullptr;
+    default: *count = (int)(sizeof(kKwayKV) / sizeof(kKwayKV[0])); return kKwayKV;
     }
 }
 

@@ -294,7 +294,7 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
 static int ilog2(uint64_t x) { return 63 - __builtin_clzll(x); }
 
 bool Ctx::kway_on(const KeyInfo &ki, uint64_t n) {
-    if (ki.kc == KC_KV || tune[ki.kc].v[TP_KWAY_FANIN] < 2) return false;
+    if (tune[ki.kc].v[TP_KWAY_FANIN] < 2) return false;
     return n > block_shape(ki.kc, TP_PLAIN_BLOCK).tile();
 }
 

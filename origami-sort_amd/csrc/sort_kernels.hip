@@ -138,8 +138,9 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
     __syncthreads();
     lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, base, lim, cout, vout);
-    if constexpr (!HV)
-        if (sp && sp->smp) write_samples<BLOCK>(sk, lim, base, sp->smp, sp->rlog2, sp->kw, sp->slog2);
+    // samples of the whole tile: in KV tiles the items past lim (maximum keys
+    // written straight to the tail) sit in LDS as maximum keys too
+    if (sp && sp->smp) write_samples<BLOCK>(sk, cnt, base, sp->smp, sp->rlog2, sp->kw, sp->slog2);
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>

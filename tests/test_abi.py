@@ -66,14 +66,13 @@ def test_shape_registry():
 
 
 def test_kway_shape_registry():
-    # K-way merge shapes exist for key-only classes only; every block is whole
-    # LDS swizzle blocks (256 items for 4-byte keys, 128 for 8-byte)
-    for kind, blk in ((ogs.U32, 256), (ogs.U64, 128)):
+    # K-way merge shapes for every class; every block is whole LDS swizzle
+    # blocks (256 items for 4-byte keys, 128 for 8-byte)
+    for kind, blk in ((ogs.U32, 256), (ogs.U64, 128), (ogs.KV_U64U64, 128)):
         ks = ogs.shapes(kind, 2)
         assert ks
         for b, ipt, w in ks:
             assert b % blk == 0 and ipt & (ipt - 1) == 0 and w == -1
-    assert ogs.shapes(ogs.KV_U64U64, 2) == []
 
 
 def test_kway_params_checked():

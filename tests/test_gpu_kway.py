@@ -155,3 +155,54 @@ def test_kway_large_checksums(ctx):
         assert bool(torch.equal(o, ref))
     finally:
         restore(ctx, ogs.U32)
+
+
+KV = (ogs.KV_U64U64, ogs.KV_I64I64)
+
+
+def check_kv(kind, got, a):
+    """Key sequence bit-exact; per key the same multiset of values (Origami is
+    not stable)."""
+    ref = oracle.sort(kind, a)
+    assert np.array_equal(got["key"], ref["key"]), "key sequence differs"
+    order_in = np.lexsort((a["value"], a["key"]))
+    order_out = np.lexsort((got["value"], got["key"]))
+    assert np.array_equal(a[order_in].view(np.uint8), got[order_out].view(np.uint8))
+
+
+@pytest.mark.parametrize("kind", KV)
+@pytest.mark.parametrize("fanin", [2, 4, 8, 16])
+@pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.ZIPF, ogs.SORTED, ogs.REVERSE, ogs.DISTINCT16])
+def test_kway_kv_kinds(ctx, kind, fanin, dist):
+    n = 200003
+    say(f"case kway kv kind={ogs.KIND_NAMES[kind]} fanin={fanin} dist={dist} n={n}")
+    ctx.tune_set(kind, "plain_block", 1)  # 128 x 8 pairs: many merge levels
+    ctx.tune_set(kind, "kway_fanin", fanin)
+    ctx.tune_set(kind, "kway_sample", 5)
+    ctx.tune_set(kind, "kway_shape", 0)
+    try:
+        a = ogs.generate(kind, dist, n, seed=21 + fanin)
+        check_kv(kind, gpu_sort(ctx, kind, a), a)
+    finally:
+        restore(ctx, kind)
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("sample", [4, 6])
+def test_kway_kv_shapes_and_max_keys(ctx, shape, sample):
+    # maximum keys (the pad key) and ties inside and across runs, values = index
+    n = 654321
+    say(f"case kway kv shape={shape} sample={sample} n={n}")
+    ctx.tune_set(ogs.KV_U64U64, "kway_fanin", 16)
+    ctx.tune_set(ogs.KV_U64U64, "kway_sample", sample)
+    ctx.tune_set(ogs.KV_U64U64, "kway_shape", shape)
+    try:
+        rng = np.random.default_rng(shape * 10 + sample)
+        a = np.zeros(n, dtype=ogs.KV_DTYPE)
+        a["key"] = rng.choice(np.array([0, 7, 2**64 - 2, 2**64 - 1], dtype=np.uint64), n)
+        a["value"] = np.arange(n, dtype=np.uint64)
+        check_kv(ogs.KV_U64U64, gpu_sort(ctx, ogs.KV_U64U64, a), a)
+        b = ogs.generate(ogs.KV_U64U64, ogs.UNIFORM, n, seed=shape + sample)
+        check_kv(ogs.KV_U64U64, gpu_sort(ctx, ogs.KV_U64U64, b), b)
+    finally:
+        restore(ctx, ogs.KV_U64U64)

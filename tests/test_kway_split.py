@@ -267,3 +267,112 @@ def test_kway_tile_bounds_hold_at_headline_shapes():
                     continue
                 worst = (q + kw) * (1 << slog2) - kw + kw * (ipt - 1)
                 assert worst <= C
+
+
+# ---- key-value form: ascending runs, exhaustion tested against real lengths
+
+def corank_asc(K, a_off, la, b_off, lb, diag, top):
+    """lds_corank_lift: A items among the first diag outputs of merge(A, B) (ties to A)."""
+    lo = max(0, diag - lb)
+    hi = min(diag, la)
+    a = lo
+    s = top
+    while s > 0:
+        t = min(a + s, hi)
+        ka = K[a_off + t - 1]
+        kb = K[b_off + diag - t]
+        a = a if kb < ka else t
+        s >>= 1
+    return a
+
+
+def merge_serial_kv(K, V, a_off, la, b_off, lb, ai, bi, cnt, ipt):
+    """lds_merge_serial<HV=true>: outputs r[0, cnt), values by position."""
+    ia, ib = a_off + ai, b_off + bi
+    ea, eb = a_off + la, b_off + lb
+    rk, rv = [KMAX64] * ipt, [0] * ipt
+    for i in range(cnt):
+        take_a = ib >= eb or (ia < ea and not (K[ib] < K[ia]))
+        p = ia if take_a else ib
+        rk[i], rv[i] = K[p], V[p]
+        if take_a:
+            ia += 1
+        else:
+            ib += 1
+    return rk, rv
+
+
+KMAX64 = 2**64 - 1
+
+
+def model_tile_kv(runs, lo, hi, block, ipt):
+    kw = len(runs)
+    C = block * ipt
+    n = [hi[j] - lo[j] for j in range(kw)]
+    P, Rr = [0], [0]
+    for j in range(kw):
+        P.append(P[-1] + (n[j] + ipt - 1) // ipt * ipt)
+        Rr.append(Rr[-1] + n[j])
+    Np = P[-1]
+    assert Np <= C
+    rng = np.random.default_rng(Np)
+    K = [int(x) for x in rng.integers(0, 2**63, C + 2 * ipt + 2)]
+    V = [int(x) for x in rng.integers(0, 2**63, C + 2 * ipt + 2)]
+    for j in range(kw):
+        for q in range(P[j + 1] - P[j]):
+            if q < n[j]:
+                K[P[j] + q], V[P[j] + q] = runs[j][lo[j] + q]
+            else:
+                K[P[j] + q], V[P[j] + q] = KMAX64, 0
+    logkw = kw.bit_length() - 1
+    for lev in range(1, logkw + 1):
+        hv = 1 << (lev - 1)
+        top = pow2_floor(max(min(Rr[s0 + hv] - Rr[s0], Rr[s0 + 2 * hv] - Rr[s0 + hv])
+                             for s0 in range(0, kw, 2 * hv)))
+        regs = {}
+        for tid in range(block):
+            c0 = tid * ipt
+            if c0 >= Np:
+                continue
+            j0 = sum(1 for j in range(1, kw) if P[j] <= c0)
+            s0 = (j0 >> lev) << lev
+            a_off, mid = P[s0], P[s0 + hv]
+            la, lb = Rr[s0 + hv] - Rr[s0], Rr[s0 + 2 * hv] - Rr[s0 + hv]
+            diag = c0 - a_off
+            c = min(max(la + lb - diag, 0), ipt)
+            rk, rv = [KMAX64] * ipt, [0] * ipt
+            if c > 0:
+                a = corank_asc(K, a_off, la, mid, lb, diag, top)
+                rk, rv = merge_serial_kv(K, V, a_off, la, mid, lb, a, diag - a, c, ipt)
+            regs[tid] = (rk, rv)
+        for tid, (rk, rv) in regs.items():
+            for i in range(ipt):
+                K[tid * ipt + i], V[tid * ipt + i] = rk[i], rv[i]
+    cnt = sum(n)
+    return list(zip(K[:cnt], V[:cnt]))
+
+
+@pytest.mark.parametrize("kw", [2, 4, 8, 16])
+@pytest.mark.parametrize("dist", ["uniform", "few", "max"])
+def test_kway_kv_tile_model(kw, dist):
+    """KV tiles: keys sorted, (key, value) multiset kept, maximum keys never
+    displaced by pads (the 'max' case mixes real maximum keys into every run)."""
+    rng = np.random.default_rng(kw + len(dist))
+    block, ipt = 64, 8
+    runs, lo, hi = [], [], []
+    for j in range(kw):
+        m = int(rng.integers(0, min(60, block * ipt // kw - ipt)))  # padded slices fit the tile
+        if dist == "uniform":
+            keys = rng.integers(0, 2**63, m)
+        elif dist == "few":
+            keys = rng.integers(0, 3, m)
+        else:
+            keys = np.where(rng.random(m) < 0.5, KMAX64, rng.integers(0, 5, m).astype(object))
+        keys = sorted(int(k) for k in keys)
+        runs.append([(k, 1000 * j + i) for i, k in enumerate(keys)])
+        lo.append(0)
+        hi.append(m)
+    out = model_tile_kv(runs, lo, hi, block, ipt)
+    allp = sorted(p for run in runs for p in run)
+    assert [k for k, _ in out] == [k for k, _ in allp]
+    assert sorted(out) == allp
