@@ -191,9 +191,9 @@ int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n
  * in the partition kernel: 1, 4, 8 or 16), "plain_fanin" (2: one merge level per
  * HBM pass; 4: two levels per pass, the last pass 2-way when one level is
  * left), "quad_lanes" (lanes per tile split in a 4-way pass: 1, 2, 4 or 8),
- * "kway_fanin" (key-only kinds: the largest fan-in of the sample-split K-way
- * merge pass, 2, 4, 8, 16 or 32, which replaces the 2- and 4-way passes of a
- * plain sort; 0 = off), "kway_shape" (index into ogs_shape_info's K-way
+ * "kway_fanin" (the largest fan-in of the sample-split K-way merge pass, 2,
+ * 4, 8, 16 or 32, which replaces the 2- and 4-way passes of a plain sort; 0 =
+ * off; structure-of-arrays KV sorts keep the 2- and 4-way passes), "kway_shape" (index into ogs_shape_info's K-way
  * list), "kway_sample" (log2 of its sample spacing, 3-10).  Block, merge and
  * K-way shape parameters take an index into ogs_shape_info's lists; *_max
  * take an item count.  Segments of up to
@@ -212,8 +212,7 @@ int ogs_tune_load(ogs_ctx *ctx, const char *path);
 
 /* Compiled kernel shapes of a key type: merge = 0 lists block-sort shapes
  * (workgroup size, items per lane, in-wave bitonic levels), merge = 1 merge
- * shapes, merge = 2 K-way merge shapes (both wlev = -1; no K-way shapes for
- * key-value kinds).  Fills entry idx when it exists; returns the count. */
+ * shapes, merge = 2 K-way merge shapes (both wlev = -1).  Fills entry idx when it exists; returns the count. */
 int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *wlev);
 
 /* Per-kernel timing with HIP events recorded on the launching stream, for

@@ -310,7 +310,7 @@ std::vector<int> Ctx::kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile) {
     return lv;
 }
 
-// Plain key-only sort with sample-split K-way passes.  Pass p merges groups of
+// Plain sort with sample-split K-way passes.  Pass p merges groups of
 // 2^lv[p] runs; its tile splits come from every 2^sl[p]-th item of each input
 // run (written as tagged samples by the kernel that wrote the run), merged per
 // group by 2-way merge passes over the sample array.  Sample buffers: S0 holds

@@ -145,7 +145,7 @@ struct Ctx {
     // values written to separate arrays by the last pass (structure of arrays)
     int sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s,
              const void *vin = nullptr, void *kout = nullptr, void *vout = nullptr);
-    // key-only plain sort with sample-split K-way merge passes (n > block tile)
+    // plain sort with sample-split K-way merge passes (n > block tile)
     int sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s);
     // merge passes of a K-way plain sort: levels per pass (largest first)
     std::vector<int> kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile);
