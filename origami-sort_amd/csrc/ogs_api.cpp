@@ -73,6 +73,7 @@ static Tuning default_tuning(KeyClass kc) {
     t.v[TP_KWAY_FANIN] = 0;
     t.v[TP_KWAY_SHAPE] = 0;
     t.v[TP_KWAY_SAMPLE] = 5;
+    t.v[TP_KWAY_SCORANK] = 0;
     return t;
 }
 
@@ -402,7 +403,8 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
             for (int j = 0; j < mcnt; j++)
                 if (mt[j].tile() <= 2 * Rs && (best < 0 || mt[j].tile() > mt[best].tile())) best = j;
             if (best < 0) return fail(OGS_ERR_ARG, "no sample merge tile divides 2^%d", (int)ilog2(2 * Rs));
-            e = mt[best].run(sa, sb[q & 1], ns, Rs, 0, mpart, lanes, s, nullptr, nullptr);
+            e = mt[best].run(sa, sb[q & 1], ns, Rs, 0, tune[ki.kc].v[TP_KWAY_SCORANK] ? nullptr : mpart, lanes, s,
+                             nullptr, nullptr);
             if (e != hipSuccess) return hip_fail(e, "sample merge launch");
             sa = sb[q & 1];
         }

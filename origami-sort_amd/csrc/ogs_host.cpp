@@ -30,7 +30,7 @@ const char *kParamNames[TP_COUNT] = {"plain_block",     "plain_merge", "seg_smal
                                      "seg_small_max",   "seg_large_block", "seg_merge",
                                      "seg_tiny_block",  "seg_tiny_max", "plain_corank", "seg_plain_min",
                                      "partition_lanes", "plain_fanin", "quad_lanes", "kway_fanin",
-                                     "kway_shape", "kway_sample"};
+                                     "kway_shape", "kway_sample", "kway_scorank"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -213,7 +213,7 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
     int cnt = 0;
     const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX || p == TP_SEG_PLAIN_MIN;
     if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
-    else if (p == TP_PLAIN_CORANK) cnt = 2;
+    else if (p == TP_PLAIN_CORANK || p == TP_KWAY_SCORANK) cnt = 2;
     else if (p == TP_PLAIN_FANIN) {
         if (value != 2 && value != 4) return fail(OGS_ERR_ARG, "plain_fanin must be 2 or 4 (got %d)", value);
         reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;

@@ -44,7 +44,8 @@ enum TuneParam {
     TP_KWAY_FANIN = 13,     // key-only plain merges: largest fan-in of the sample-split K-way pass (0 = off)
     TP_KWAY_SHAPE = 14,     // tile shape of the K-way merge kernel
     TP_KWAY_SAMPLE = 15,    // log2 of the sample spacing of the K-way pass
-    TP_COUNT = 16
+    TP_KWAY_SCORANK = 16,   // sample merges of the K-way pass: 0 co-rank table kernel, 1 co-ranks inside the merge kernel
+    TP_COUNT = 17
 };
 struct Tuning {
     int v[TP_COUNT];
