@@ -200,9 +200,17 @@ template <int M> __device__ __forceinline__ uint64_t xshfl(uint64_t x) {
 // own item, so no item is duplicated or lost.
 template <bool HV, typename K>
 __device__ __forceinline__ void lane_pair_keep(K &mine, uint64_t &vm, K other, uint64_t vo, bool upper) {
-    bool keep = upper ? !(mine < other) : !(other < mine);
-    mine = keep ? mine : other;
-    if constexpr (HV) vm = keep ? vm : vo;
+    if constexpr (!HV) {
+        // key-only: equal keys are identical bits, so the pair is min / max
+        const K lo = other < mine ? other : mine;
+        const K hi = other < mine ? mine : other;
+        mine = upper ? hi : lo;
+        (void)vm; (void)vo;
+    } else {
+        bool keep = upper ? !(mine < other) : !(other < mine);
+        mine = keep ? mine : other;
+        vm = keep ? vm : vo;
+    }
 }
 
 // In-lane bitonic half-cleaners: distances IPT/2 .. 1.
