@@ -222,7 +222,7 @@ def bench_generator(ctx, buf, reps=5):
     return res
 
 
-def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5):
+def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5, coalesce=0):
     """Serving loop: Poisson arrivals of ORDER-BY requests (key type uniform over
     u32/u64/u64+u64, length log-uniform in [2^10, 2^26], keys uniform or Zipf 50/50,
     device-resident), served by ogs_server (continuous batching, 4 streams) and,
@@ -282,7 +282,7 @@ def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5):
     # the server at the same arrival times
     dsts = [torch.empty_like(src) for _, _, src in reqs]
     budget = 3 * max(n * ogs.WIDTH[k] for k, n, _ in reqs) + (256 << 20)
-    srv = ogs.Server(ctx, hbm_budget_bytes=int(budget), streams=streams)
+    srv = ogs.Server(ctx, hbm_budget_bytes=int(budget), streams=streams, coalesce_max_items=coalesce)
     torch.cuda.synchronize()
     tickets = []
     t0 = time.perf_counter()
@@ -336,6 +336,8 @@ def main():
     ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--serving-load", type=float, default=1.5,
                     help="offered request rate as a multiple of the serial plain-launch capacity")
+    ap.add_argument("--serving-coalesce", type=int, default=0,
+                    help="longest request the server batches with others (0: the server's default)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after timing, run three 1-input-sized device copies (torch copy_) as the "
                          "known-size reference kernel that calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE")
@@ -472,7 +474,8 @@ def main():
         line["batched"] = bench_batched(ctx, args.batched_log2_total, 20, nseg_fixed=args.batched_segments)
     if not args.no_serving:
         torch.cuda.empty_cache()
-        line["serving"] = bench_serving(ctx, args.serving_requests, args.serving_load, args.serving_streams)
+        line["serving"] = bench_serving(ctx, args.serving_requests, args.serving_load, args.serving_streams,
+                                        coalesce=args.serving_coalesce)
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
         import oracle
