@@ -223,8 +223,18 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
     for (int b = KW / 2; b >= 1; b >>= 1) j0 += s_P[j0 + b] <= c0 ? b : 0;
     // chunk source: position c0 + r holds item base + r (ascending slices) or
     // base - r (key-only odd slices, the B side of the first level, stored
-    // descending); real items are r in [v0, v1), the rest are pads
-    uint32_t cb_lo = 0, cb_hi = 0, cmeta = 0;
+    // descending); real items are r in [v0, v1), the rest are pads.
+    // Narrow form (every item of the group lies within 4 GiB of its first):
+    // loads take the group's address as a uniform base plus a 32-bit byte
+    // offset; a chunk is sent as cb_lo = its byte offset B minus d, cb_hi = d
+    // (-1 descending, else 0) and cmeta = the mask of its pad positions, so
+    // item r sits at ((r * ES) ^ d) + (B - d) = B -/+ r * ES.  Wide form:
+    // cb_lo / cb_hi = the 64-bit item index base, cmeta = v0 | v1 << 8 |
+    // desc << 16.
+    constexpr uint32_t ES = HV ? 16u : (uint32_t)sizeof(K);  // bytes per record
+    const uint64_t gend = gbase + KW * R < n ? gbase + KW * R : n;
+    const bool narrow = (gend - gbase) * ES <= (1ull << 32);  // uniform
+    uint32_t cb_lo = 0, cb_hi = 0, cmeta = narrow ? ~0u : 0u;
     if (live) {
         const int pj = s_P[j0], npj = s_P[j0 + 1] - pj, nj = s_n[j0];
         const uint64_t src = s_src[j0];
@@ -243,9 +253,16 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
             v1 = nj - q;
             v1 = v1 < 0 ? 0 : v1 > IPT ? IPT : v1;
         }
-        cb_lo = (uint32_t)base;
-        cb_hi = (uint32_t)(base >> 32);
-        cmeta = (uint32_t)(v0 | (v1 << 8) | ((int)desc << 16));
+        if (narrow) {
+            const uint32_t d = desc ? ~0u : 0u;
+            cb_lo = (uint32_t)((base - gbase) * ES) - d;
+            cb_hi = d;
+            cmeta = ~(uint32_t)(((1ull << v1) - 1) & ~((1ull << v0) - 1));
+        } else {
+            cb_lo = (uint32_t)base;
+            cb_hi = (uint32_t)(base >> 32);
+            cmeta = (uint32_t)(v0 | (v1 << 8) | ((int)desc << 16));
+        }
     }
     {
         // Wave w loads LDS positions [w * 64 * IPT, (w + 1) * 64 * IPT), the
@@ -262,26 +279,50 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
         const int wbase = (tid >> 6) * 64 * IPT;
         K kk[IPT];
         uint64_t vv[IPT];
+        if (narrow) {
+            const char *gb = (const char *)in + gbase * ES;  // uniform: scalar registers
+            const uint32_t rb = (uint32_t)r * ES;
 #pragma unroll
-        for (int i = 0; i < IPT; i++) {
-            const int pw = BLK * (i % NB) + 64 * (i / NB) + lane;  // position within the wave's range
-            const int sl = (pw / IPT) << 2;
-            const uint32_t blo = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_lo);
-            const uint32_t bhi = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_hi);
-            const uint32_t m = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cmeta);
-            const bool real = r >= (int)(m & 255) && r < (int)((m >> 8) & 255);
-            // branch-free: offset +r or -r (conditional negate by the sign
-            // mask), then one select of the address
-            const int neg = -(int)(m >> 16);
-            const int64_t off = (int64_t)((r ^ neg) - neg);
-            const uint64_t b = ((uint64_t)bhi << 32) | blo;
-            const uint64_t mask = (uint64_t)0 - (uint64_t)real;
-            const uint64_t src = gbase + ((b + (uint64_t)off - gbase) & mask);
-            K x;
-            uint64_t xv = 0;
-            GMem<K, HV>::load(in, src, x, xv);
-            kk[i] = real ? x : KMAX;
-            vv[i] = real ? xv : 0;
+            for (int i = 0; i < IPT; i++) {
+                const int pw = BLK * (i % NB) + 64 * (i / NB) + lane;  // position within the wave's range
+                const int sl = (pw / IPT) << 2;
+                const uint32_t bb = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_lo);
+                const uint32_t d = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_hi);
+                const uint32_t pm = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cmeta);
+                // -1 for a pad, else 0: a pad reads the group's first item and
+                // becomes the maximum key (value 0)
+                const int p = __builtin_amdgcn_sbfe((int)pm, (uint32_t)r, 1u);
+                const uint32_t off = ((rb ^ d) + bb) & ~(uint32_t)p;
+                if constexpr (HV) {
+                    const ulonglong2 t = *(const ulonglong2 *)(gb + off);
+                    kk[i] = (K)t.x | (K)(int64_t)p;
+                    vv[i] = t.y & ~(uint64_t)(int64_t)p;
+                } else {
+                    kk[i] = *(const K *)(gb + off) | (K)(typename std::make_signed<K>::type)p;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < IPT; i++) {
+                const int pw = BLK * (i % NB) + 64 * (i / NB) + lane;
+                const int sl = (pw / IPT) << 2;
+                const uint32_t blo = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_lo);
+                const uint32_t bhi = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_hi);
+                const uint32_t m = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cmeta);
+                const bool real = r >= (int)(m & 255) && r < (int)((m >> 8) & 255);
+                // branch-free: offset +r or -r (conditional negate by the sign
+                // mask), then one select of the address
+                const int neg = -(int)(m >> 16);
+                const int64_t off = (int64_t)((r ^ neg) - neg);
+                const uint64_t b = ((uint64_t)bhi << 32) | blo;
+                const uint64_t mask = (uint64_t)0 - (uint64_t)real;
+                const uint64_t src = gbase + ((b + (uint64_t)off - gbase) & mask);
+                K x;
+                uint64_t xv = 0;
+                GMem<K, HV>::load(in, src, x, xv);
+                kk[i] = real ? x : KMAX;
+                vv[i] = real ? xv : 0;
+            }
         }
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
@@ -393,9 +434,9 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     return hipGetLastError();
 }
 
-#define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16) X(1024, 32)
-#define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8) X(1024, 16)
-#define OGS_KWAY_SHAPES_KV(X) X(512, 8) X(256, 16) X(256, 8) X(512, 4)
+#define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16) X(1024, 32) X(1024, 16)
+#define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8) X(1024, 16) X(1024, 8)
+#define OGS_KWAY_SHAPES_KV(X) X(512, 8) X(256, 16) X(256, 8) X(512, 4) X(1024, 4)
 #define OGS_KX(K, HV, B, I)                                                                                 \
     {B, I, {&launch_kway<K, HV, 2, B, I>, &launch_kway<K, HV, 4, B, I>, &launch_kway<K, HV, 8, B, I>,       \
             &launch_kway<K, HV, 16, B, I>, &launch_kway<K, HV, 32, B, I>}},

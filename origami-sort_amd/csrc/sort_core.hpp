@@ -115,6 +115,11 @@ struct GMem {
 #ifndef OGS_REV_MERGE
 #define OGS_REV_MERGE 1
 #endif
+// 1: the full-slice merge on the bitonic layout keeps (loser, newcomer) heads
+// instead of (A head, B head).
+#ifndef OGS_MERGE_LOSER
+#define OGS_MERGE_LOSER 1
+#endif
 template <typename T> __device__ __forceinline__ int lsw(int i) {
 #if OGS_LDS_SWIZZLE
     if constexpr (sizeof(T) == 4) return i ^ (((i >> 5) & 7) << 2);
@@ -437,6 +442,30 @@ __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i
         constexpr int W = (int)sizeof(K);
         const int dlt = (ib - ia) * W;
         int pa = ia * W;
+#if OGS_MERGE_LOSER
+        // Key-only, so equal keys are identical bits and ties may go either
+        // way: the registers hold the head that lost the last compare (L) and
+        // the item read last (N, from run A when nA), the output is min(N, L),
+        // the loser max(N, L), and the winner's run is a lane mask (scalar
+        // instructions).  Saves the two head selects of the form below.
+        K N = ka, L = kb;
+        bool nA = true;
+#pragma unroll
+        for (int i = 0; i < IPT; i++) {
+            const bool wN = !(L < N);  // N wins (ties: either)
+            const K lo = wN ? N : L;
+            L = wN ? L : N;
+            r[i] = lo;
+            const bool fromA = wN == nA;  // the winner's run
+            const int pa1 = pa + W;
+            const int nb = pa + dlt - W * (i + 1);
+            const int nx = fromA ? pa1 : nb;
+            pa = fromA ? pa1 : pa;
+            N = lds_at_bytes(sk, nx);
+            nA = fromA;
+        }
+        return;
+#else
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
             const bool takeA = !(kb < ka);
@@ -450,6 +479,7 @@ __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i
             kb = takeA ? kb : nk;
         }
         return;
+#endif
     }
 #pragma unroll
     for (int i = 0; i < IPT; i++) {

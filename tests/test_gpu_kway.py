@@ -85,7 +85,7 @@ def test_kway_sizes(ctx, n, fanin):
 
 @pytest.mark.parametrize("kind", [ogs.U32, ogs.U64])
 @pytest.mark.parametrize("sample", [3, 4, 6, 7])
-@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4, 5])
 def test_kway_shapes_and_spacing(ctx, kind, sample, shape):
     n = 777777
     say(f"case kway kind={ogs.KIND_NAMES[kind]} sample={sample} shape={shape} n={n}")
@@ -187,7 +187,7 @@ def test_kway_kv_kinds(ctx, kind, fanin, dist):
         restore(ctx, kind)
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("sample", [4, 6])
 def test_kway_kv_shapes_and_max_keys(ctx, shape, sample):
     # maximum keys (the pad key) and ties inside and across runs, values = index
