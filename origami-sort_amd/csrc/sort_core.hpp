@@ -184,13 +184,20 @@ __device__ __forceinline__ void lane_network(K (&r)[N], uint64_t (&v)[N]) {
 // DPP row_half_mirror (xor 7) and row_mirror (xor 15), ds_swizzle bitmask mode
 // for the other masks below 32, ds_bpermute beyond.
 // ----------------------------------------------------------------------------
+// The DPP forms are written as update_dpp with bound_ctrl set: every lane has
+// a source under these patterns, so the zero fill never applies, and the
+// compiler can then fold the move into the min / max that consumes it
+// (v_min_u32_dpp), one VALU instruction fewer per item per exchange.
+template <int CTRL> __device__ __forceinline__ uint32_t dpp32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, true);
+}
 template <int M>
 __device__ __forceinline__ uint32_t xshfl32(uint32_t x) {
-    if constexpr (M == 1) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);
-    else if constexpr (M == 2) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);
-    else if constexpr (M == 3) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xF, 0xF, false);
-    else if constexpr (M == 7) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x141, 0xF, 0xF, false);
-    else if constexpr (M == 15) return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x140, 0xF, 0xF, false);
+    if constexpr (M == 1) return dpp32<0xB1>(x);
+    else if constexpr (M == 2) return dpp32<0x4E>(x);
+    else if constexpr (M == 3) return dpp32<0x1B>(x);
+    else if constexpr (M == 7) return dpp32<0x141>(x);
+    else if constexpr (M == 15) return dpp32<0x140>(x);
     else if constexpr (M < 32) return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (M << 10) | 0x1F);
     else return (uint32_t)__shfl_xor((int)x, M, 64);
 }
@@ -252,16 +259,22 @@ __device__ __forceinline__ void cross_cleaners(K (&r)[IPT], uint64_t (&v)[IPT], 
 template <int G, int IPT, bool HV, typename K>
 __device__ __forceinline__ void wave_merge_level(K (&r)[IPT], uint64_t (&v)[IPT], int lane) {
     const bool upper = (lane & (G / 2)) != 0;
-    K pk[IPT];
-    uint64_t pv[IPT];
+    // each exchange reads the partner's item from a copy of the registers and
+    // is issued next to the min / max that use it, so the compiler can fold
+    // the DPP move into them
+    K q[IPT];
+    uint64_t qv[IPT];
 #pragma unroll
     for (int i = 0; i < IPT; i++) {
-        pk[i] = xshfl<G - 1>(r[IPT - 1 - i]);
-        if constexpr (HV) pv[i] = xshfl<G - 1>(v[IPT - 1 - i]);
-        else pv[i] = 0;
+        q[i] = r[i];
+        qv[i] = HV ? v[i] : 0;
     }
 #pragma unroll
-    for (int i = 0; i < IPT; i++) lane_pair_keep<HV>(r[i], v[i], pk[i], pv[i], upper);
+    for (int i = 0; i < IPT; i++) {
+        const K pk = xshfl<G - 1>(q[IPT - 1 - i]);
+        const uint64_t pv = HV ? xshfl<G - 1>(qv[IPT - 1 - i]) : 0;
+        lane_pair_keep<HV>(r[i], v[i], pk, pv, upper);
+    }
     cross_cleaners<G / 4, IPT, HV>(r, v, lane);
     lane_cleaners<IPT, HV>(r, v);
 }
