@@ -891,10 +891,10 @@ static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R,
 #define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8) X(1024, 16) \
     X(512, 32) X(128, 4)
 #define OGS_BLOCK_SHAPES_64(X) X(256, 16, 1) X(256, 8, 1) X(128, 16, 1) X(512, 8, 1) X(512, 16, 1) \
-    X(1024, 16, 1) X(64, 16, 1) X(64, 8, 1)
+    X(1024, 16, 1) X(64, 16, 1) X(64, 8, 1) X(512, 16, 2) X(512, 16, 3)
 #define OGS_MERGE_SHAPES_64(X) X(256, 16) X(256, 8) X(512, 8) X(128, 16) X(512, 16) X(1024, 8) X(128, 4)
 #define OGS_BLOCK_SHAPES_KV(X) X(256, 8, 1) X(128, 8, 1) X(256, 4, 1) X(512, 4, 1) X(512, 8, 1) X(64, 8, 1) \
-    X(64, 4, 1)
+    X(64, 4, 1) X(512, 8, 2) X(512, 8, 3)
 #define OGS_MERGE_SHAPES_KV(X) X(256, 8) X(128, 8) X(256, 4) X(512, 4) X(512, 8) X(1024, 4) X(128, 4)
 
 #define OGS_BX(K, HV, B, I, W) {B, I, W, &launch_block<K, HV, B, I, W>, &launch_seg_block<K, HV, B, I, W>},
