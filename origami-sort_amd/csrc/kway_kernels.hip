@@ -128,6 +128,39 @@ __global__ void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2
     tab[t * KW + j] = split;
 }
 
+// Inclusive sums across lanes 0 .. KW-1 of a wave (KW <= 32): DPP row shifts
+// give the sums within each row of 16 lanes (lanes with no source add 0); the
+// lanes of the second row then add the first row's total, lane 15's sum.
+template <int D> __device__ __forceinline__ uint32_t lane_shr(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x110 + D, 0xF, 0xF, true);
+}
+template <int KW> __device__ __forceinline__ uint32_t lane_scan(uint32_t x) {
+    if constexpr (KW > 1) x += lane_shr<1>(x);
+    if constexpr (KW > 2) x += lane_shr<2>(x);
+    if constexpr (KW > 4) x += lane_shr<4>(x);
+    if constexpr (KW > 8) x += lane_shr<8>(x);
+    if constexpr (KW > 16) {
+        const uint32_t u = (uint32_t)__shfl((int)x, 15, 64);
+        x += (threadIdx.x & 63) >= 16 ? u : 0u;
+    }
+    return x;
+}
+template <int D> __device__ __forceinline__ uint64_t lane_shr64(uint64_t x) {
+    return ((uint64_t)lane_shr<D>((uint32_t)(x >> 32)) << 32) | lane_shr<D>((uint32_t)x);
+}
+template <int KW> __device__ __forceinline__ uint64_t lane_scan64(uint64_t x) {
+    if constexpr (KW > 1) x += lane_shr64<1>(x);
+    if constexpr (KW > 2) x += lane_shr64<2>(x);
+    if constexpr (KW > 4) x += lane_shr64<4>(x);
+    if constexpr (KW > 8) x += lane_shr64<8>(x);
+    if constexpr (KW > 16) {
+        const uint64_t u = ((uint64_t)(uint32_t)__shfl((int)(x >> 32), 15, 64) << 32) |
+                           (uint32_t)__shfl((int)(uint32_t)x, 15, 64);
+        x += (threadIdx.x & 63) >= 16 ? u : 0ull;
+    }
+    return x;
+}
+
 // Items tid + k * BLOCK (k < IPT) of an LDS tile (sp0 = its swizzled base) to
 // ob[tid + k * BLOCK], with output codec CODEC: all reads, then all stores.
 template <int BLOCK, int IPT, int CODEC, typename K>
@@ -168,46 +201,57 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
     const uint64_t g = t / ntg, tl = t % ntg;
     const uint64_t gbase = g * KW * R;
     const uint64_t nt_here = gbase + KW * R >= n ? nt_last : ntg;
-    if (tid < KW) {
-        const uint64_t lenj = kw_run_len(n, gbase, R, tid);
-        const uint64_t lo = tab[t * KW + tid];
-        const uint64_t hi = tl + 1 < nt_here ? tab[(t + 1) * KW + tid] : lenj;
-        s_src[tid] = gbase + (uint64_t)tid * R + lo;
-        // a split table that is not monotone gives a negative count: the tile is skipped
-        s_n[tid] = hi >= lo && hi <= lenj ? (int)(hi - lo) : -1;
+    if (tid < 64) {
+        // wave 0: lane j < KW takes slice j, and the prefix sums over the
+        // slices are inclusive scans across the lanes
+        const int j = tid;
+        int nj = 0;
+        uint64_t lo = 0;
+        if (j < KW) {
+            const uint64_t lenj = kw_run_len(n, gbase, R, j);
+            lo = tab[t * KW + j];
+            const uint64_t hi = tl + 1 < nt_here ? tab[(t + 1) * KW + j] : lenj;
+            s_src[j] = gbase + (uint64_t)j * R + lo;
+            // a split table that is not monotone gives a negative count: the tile is skipped
+            nj = hi >= lo && hi <= lenj ? (int)(hi - lo) : -1;
+            s_n[j] = nj;
+        }
+        const bool bad = __ballot(nj < 0) != 0;
+        const uint32_t p = lane_scan<KW>(nj > 0 ? (uint32_t)((nj + IPT - 1) & ~(IPT - 1)) : 0u);
+        const uint32_t c = lane_scan<KW>(nj > 0 ? (uint32_t)nj : 0u);
+        const uint64_t o = lane_scan64<KW>(lo);  // o0 = gbase + the sum of the start splits
+        if (j < KW) {
+            s_P[j + 1] = (int)p;
+            s_R[j + 1] = (int)c;
+        }
+        if (j == 0) {
+            s_P[0] = 0;
+            s_R[0] = 0;
+        }
+        if (j == KW - 1) {
+            s_o0 = gbase + o;
+            s_cnt = bad || (int)p > C ? -1 : (int)c;  // cannot happen with a consistent table
+        }
     }
     __syncthreads();
-    if (tid == 0) {
-        int p = 0, cnt = 0;
-        bool bad = false;
-        uint64_t o0 = gbase;
-        s_P[0] = 0;
-        s_R[0] = 0;
-        for (int j = 0; j < KW; j++) {
-            const int nj = s_n[j];
-            bad |= nj < 0;
-            cnt += nj;
-            p += (nj + IPT - 1) & ~(IPT - 1);
-            s_P[j + 1] = p;
-            s_R[j + 1] = cnt;
-            o0 += s_src[j] - (gbase + (uint64_t)j * R);
-        }
-        s_o0 = o0;
-        s_cnt = bad || p > C ? -1 : cnt;  // cannot happen with a consistent table
+    if (tid >= 1 && tid <= LOGKW) {
         // the co-rank of a pair (la, lb) needs pow2_floor(min(la, lb)) lifting
         // steps (KV: the real lengths); every thread of a level takes the
         // largest, so the step loop is uniform (more steps than a pair needs
-        // are harmless)
-        for (int lev = 1; lev <= LOGKW; lev++) {
-            int mx = 0;
-            for (int s0 = 0; s0 < KW; s0 += 2 << (lev - 1)) {
-                const int hv = 1 << (lev - 1);
-                const int *Q = HV ? s_R : s_P;
-                const int la = Q[s0 + hv] - Q[s0], lb = Q[s0 + 2 * hv] - Q[s0 + hv];
-                mx = max(mx, min(la, lb));
-            }
-            s_top[lev] = pow2_floor(mx);
+        // are harmless).  Lane lev computes level lev's bound.
+        // Pair k of the level spans slices [k * 2hv, (k + 1) * 2hv); indices
+        // are clamped to KW, so the pairs past the last are empty and every
+        // read is independent of the others.
+        const int lev = tid, hv = 1 << (lev - 1);
+        const int *Q = HV ? s_R : s_P;
+        int mx = 0;
+#pragma unroll
+        for (int k = 0; k < KW / 2; k++) {
+            const int a0 = min(k * 2 * hv, KW), a1 = min(a0 + hv, KW), a2 = min(a0 + 2 * hv, KW);
+            const int la = Q[a1] - Q[a0], lb = Q[a2] - Q[a1];
+            mx = max(mx, min(la, lb));
         }
+        s_top[lev] = pow2_floor(mx);
     }
     __syncthreads();
     const int cnt = s_cnt;
