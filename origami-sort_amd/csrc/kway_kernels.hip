@@ -73,41 +73,98 @@ uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int
 }
 
 // ----------------------------------------------------------------------------
-// Tile splits: KW lanes per tile (lane j: run j); tab[t * KW + j] = the split
-// of run j at tile t's first item.  Each lane runs two binary searches: over
-// run j's samples (which number precede the boundary), then over the s items
-// that separate two of them.
+// Tile splits: one wave per tile; tab[t * KW + j] = the split of run j at tile
+// t's first item, the boundary sample b = (v, r, m) at merged position p.  The
+// number of run j's samples that precede b is the index of run j's first sample
+// at or after p in the merged order (run j's samples appear there in index
+// order), so the wave reads the 8 * KW merged samples from p on, and lane j
+// takes run j's first one by ballot.  A run with no sample in that window has
+// all its samples before p when the window reaches the group's end, else lane j
+// finds the count by a binary search over run j's own samples.  A second binary
+// search then places b among the s items that separate two samples of run j.
 // ----------------------------------------------------------------------------
 template <typename K, bool HV, int KW>
-__global__ void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2, int slog2, uint64_t Q, uint64_t ntg,
-                              uint64_t ntiles, const void *__restrict__ s0, const void *__restrict__ sm,
-                              uint64_t *__restrict__ tab) {
-    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t t = gt / KW;
-    const int j = (int)(gt % KW);
-    if (t >= ntiles) return;
+__global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2, int slog2,
+                                                     uint64_t Q, uint64_t ntg, uint64_t ntiles,
+                                                     const void *__restrict__ s0, const void *__restrict__ sm,
+                                                     uint64_t *__restrict__ tab) {
+    constexpr int NW = KW * 8 / 64 > 1 ? KW * 8 / 64 : 1;  // window samples per lane
+    const int lane = threadIdx.x & 63;
+    const uint64_t t = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (t >= ntiles) return;  // uniform in the wave
     const uint64_t R = 1ull << rlog2;
     const uint64_t g = t / ntg, tl = t % ntg;
     const uint64_t gbase = g * KW * R;
+    if (tl == 0) {
+        if (lane < KW) tab[t * KW + lane] = 0;
+        return;
+    }
+    const uint64_t gs0 = gbase >> slog2;  // the group's first merged sample
+    const uint64_t gs_end = gs0 + (gbase + KW * R >= n ? kw_last_samples(n, rlog2, slog2, KW) : (KW * R) >> slog2);
+    const uint64_t p = gs0 + tl * Q;
+    int rr[NW];
+    uint32_t mm[NW];
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        const uint64_t idx = p + (uint64_t)(lane + 64 * i);
+        rr[i] = KW;  // no run
+        mm[i] = 0;
+        if (idx < gs_end) {
+            K v;
+            int r;
+            uint64_t m;
+            Sample<K>::get(sm, idx, v, r, m);
+            rr[i] = r;
+            mm[i] = (uint32_t)m;
+        }
+    }
+    // lane j: the window position of run j's first sample (-1: none)
+    int src = -1;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+#pragma unroll
+        for (int j = 0; j < KW; j++) {
+            const uint64_t b = __ballot(rr[i] == j);
+            if (lane == j && src < 0 && b != 0) src = i * 64 + (__ffsll((unsigned long long)b) - 1);
+        }
+    }
+    uint32_t mfound = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++) {
+        const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute((src >= 0 ? (src & 63) : 0) << 2, (int)mm[i]);
+        if (src >= 0 && (src >> 6) == i) mfound = x;
+    }
+    const bool to_end = p + 64 * NW >= gs_end;
+    if (lane >= KW) return;
+    const int j = lane;
     const uint64_t lenj = kw_run_len(n, gbase, R, j);
     uint64_t split = 0;
-    if (tl > 0 && lenj > 0) {
+    if (lenj > 0) {
         K v;
         int r;
         uint64_t m;
-        Sample<K>::get(sm, (gbase >> slog2) + tl * Q, v, r, m);
+        Sample<K>::get(sm, p, v, r, m);
         if (j == r) {
             split = m << slog2;
         } else {
             // run j's items precede b: those <= v when j < r, those < v when j > r
             const bool up = j < r;
-            const uint64_t sb = (gbase + (uint64_t)j * R) >> slog2;
-            uint64_t lo = 0, hi = (lenj + (1ull << slog2) - 1) >> slog2;
-            while (lo < hi) {
-                const uint64_t mid = (lo + hi) >> 1;
-                const K k = Sample<K>::key(s0, sb + mid);
-                if (up ? !(v < k) : k < v) lo = mid + 1;
-                else hi = mid;
+            const uint64_t ns_j = (lenj + (1ull << slog2) - 1) >> slog2;
+            uint64_t lo;
+            if (src >= 0) {
+                lo = mfound;
+            } else if (to_end) {
+                lo = ns_j;
+            } else {
+                const uint64_t sb = (gbase + (uint64_t)j * R) >> slog2;
+                uint64_t hi = ns_j;
+                lo = 0;
+                while (lo < hi) {
+                    const uint64_t mid = (lo + hi) >> 1;
+                    const K k = Sample<K>::key(s0, sb + mid);
+                    if (up ? !(v < k) : k < v) lo = mid + 1;
+                    else hi = mid;
+                }
             }
             if (lo > 0) {
                 // sample lo-1 precedes b and sample lo (if any) does not
@@ -468,8 +525,7 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     const uint64_t ntg = kw_group_tiles(GR >> slog2, Q);
     const uint64_t nt_last = kw_group_tiles(kw_last_samples(n, rlog2, slog2, KW), Q);
     const uint64_t ntiles = (n + GR - 1) / GR * ntg - ntg + nt_last;
-    const uint64_t threads = ntiles * KW;
-    hipLaunchKernelGGL((k_kway_bounds<K, HV, KW>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, in, n,
+    hipLaunchKernelGGL((k_kway_bounds<K, HV, KW>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, s, in, n,
                        rlog2, slog2, Q, ntg, ntiles, s0, sm, tab);
     if (mid) (void)hipEventRecord(mid, s);
     const SampleSpec none = {nullptr, 0, 0, 0};

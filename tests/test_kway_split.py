@@ -1,7 +1,10 @@
 """CPU model of the sample-split K-way merge pass (origami-sort_amd/csrc/kway_kernels.hip).
 
 The model follows the kernels step by step -- tagged samples every s items of each
-run, the per-group sample merge, the two-stage split search of k_kway_bounds, and
+run, the per-group sample merge, k_kway_bounds' splits (each run's sample count
+from the window of merged samples after the boundary, else a binary search over
+the run's samples, then a search among s items; checked against the binary
+search alone), and
 k_kway_merge's padded LDS layout (odd slices descending), per-thread co-ranks on
 the bitonic layout and level-by-level pair merges -- and checks the invariants
 the pass relies on: splits are monotone and exact (every item of a tile precedes
@@ -52,6 +55,36 @@ def split(X, v, up, slog2, samples_j):
         else:
             b = mid
     return a
+
+
+def split_window(X, v, r, j, slog2, samples_j, tags, p, kw):
+    """k_kway_bounds as the kernel runs it: the count of run j's samples that
+    precede the boundary (merged position p) is the index of run j's first
+    sample in the window of 8 * kw merged samples from p; with none there, all
+    of them when the window reaches the group's end, else the binary search."""
+    kw_window = max(64, 8 * kw)
+    window = tags[p:p + kw_window]
+    lo = None
+    for (_, rj, mj) in window:
+        if rj == j:
+            lo = mj
+            break
+    if lo is None and p + kw_window >= len(tags):
+        lo = len(samples_j)
+    if lo is None:
+        return split(X, v, j < r, slog2, samples_j), "search"
+    if lo == 0:
+        return 0, "window"
+    up = j < r
+    a, b = ((lo - 1) << slog2) + 1, min(lo << slog2, len(X))
+    while a < b:
+        mid = (a + b) // 2
+        k = int(X[mid])
+        if (not (v < k)) if up else (k < v):
+            a = mid + 1
+        else:
+            b = mid
+    return a, "window"
 
 
 def pow2_floor(x):
@@ -164,6 +197,9 @@ def model_tile(runs, lo, hi, block, ipt):
     return L[:cnt]
 
 
+PATHS = {}  # how the model's splits were found: by the window or the fallback search
+
+
 def model_pass(keys, R, kw, slog2, block, ipt):
     """One K-way pass over keys (runs of R sorted): returns the output."""
     nitems = len(keys)
@@ -193,7 +229,11 @@ def model_pass(keys, R, kw, slog2, block, ipt):
                 elif j == r:
                     row.append(m << slog2)
                 else:
-                    row.append(split(runs[j], v, j < r, slog2, smp[j]))
+                    ref = split(runs[j], v, j < r, slog2, smp[j])
+                    got, how = split_window(runs[j], v, r, j, slog2, smp[j], tags, tl * Q, kw)
+                    assert got == ref, (tl, j, got, ref, how)
+                    PATHS[how] = PATHS.get(how, 0) + 1
+                    row.append(got)
             bounds.append(row)
         bounds.append([len(X) for X in runs])
         o = 0
