@@ -305,14 +305,17 @@ template <typename K>
 __device__ __forceinline__ int lds_corank_lift(const K *sk, int a_off, int la, int b_off, int lb, int diag, int top) {
     const int lo = diag > lb ? diag - lb : 0;
     const int hi = diag < la ? diag : la;
-    int a = lo;
-    for (int s = top; s > 0; s >>= 1) {
-        const int t = min(a + s, hi);
-        const K ka = sk[lsw<K>(a_off + t - 1)];
-        const K kb = sk[lsw<K>(b_off + diag - t)];
-        a = kb < ka ? a : t;
+    // in bytes: A[t-1] at pa + t*W, B[diag-t] at pb - t*W
+    constexpr int W = (int)sizeof(K);
+    const int pa = (a_off - 1) * W, pb = (b_off + diag) * W, hib = hi * W;
+    int ab = lo * W;
+    for (int s = top * W; s >= W; s >>= 1) {
+        const int tb = min(ab + s, hib);
+        const K ka = lds_at_bytes(sk, pa + tb);
+        const K kb = lds_at_bytes(sk, pb - tb);
+        ab = kb < ka ? ab : tb;
     }
-    return a;
+    return ab / W;
 }
 
 // Items of LDS guard space around a merge buffer: one 128-byte row before it
@@ -391,6 +394,29 @@ __device__ __forceinline__ void lds_merge_serial(const K *sk, const uint64_t *sv
     } else {
         // the merge carries each output's LDS position; values are gathered
         // once afterwards, off the dependency chain of the key merge
+        if constexpr (FULL) {
+            // pointers and positions as byte offsets (keys and values are
+            // both 8 bytes, so one offset addresses both arrays)
+            constexpr int W = (int)sizeof(K);
+            int pa = ia * W, pb = ib * W;
+            const int eab = ea * W, ebb = eb * W;
+            K ka = lds_at_bytes(sk, pa), kb = lds_at_bytes(sk, pb);
+            int pos[IPT];
+#pragma unroll
+            for (int i = 0; i < IPT; i++) {
+                const bool takeA = pb >= ebb || (pa < eab && !(kb < ka));
+                r[i] = takeA ? ka : kb;
+                pos[i] = takeA ? pa : pb;
+                pa += takeA ? W : 0;
+                pb += takeA ? 0 : W;
+                const K nk = lds_at_bytes(sk, takeA ? pa : pb);
+                ka = takeA ? nk : ka;
+                kb = takeA ? kb : nk;
+            }
+#pragma unroll
+            for (int i = 0; i < IPT; i++) v[i] = lds_at_bytes(sv, pos[i]);
+            return;
+        }
         K ka = sk[lsw<K>(ia)], kb = sk[lsw<K>(ib)];
         int pos[IPT];
 #pragma unroll
