@@ -79,10 +79,37 @@ uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int
 // at or after p in the merged order (run j's samples appear there in index
 // order), so the wave reads the 8 * KW merged samples from p on, and lane j
 // takes run j's first one by ballot.  A run with no sample in that window has
-// all its samples before p when the window reaches the group's end, else lane j
-// finds the count by a binary search over run j's own samples.  A second binary
-// search then places b among the s items that separate two samples of run j.
+// all its samples before p when the window reaches the group's end, else the
+// count comes from a search over run j's own samples.  A second search then
+// places b among the s items that separate two samples of run j.  Each search
+// is run by the G = 64 / KW lanes of run j together (group_search).
 // ----------------------------------------------------------------------------
+
+// The first position in [lo, hi) whose item does not satisfy pre() (hi if all
+// do), for a predicate that holds on a prefix of the range: a (G+1)-ary search
+// by the G lanes of each lane group (lanes [q*G, q*G + G) search one range), one
+// probe per lane per step.  Every lane of the wave calls it (a group with
+// lo == hi has nothing to do); each step narrows a group's range to the gap
+// between its last probe that holds and its first that does not.
+template <int G, typename F>
+__device__ __forceinline__ uint64_t group_search(uint64_t lo, uint64_t hi, int lane, F pre) {
+    const int q = lane / G, g = lane % G;
+    const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (q * G);
+    while (__ballot(lo < hi) != 0) {
+        const uint64_t w = hi - lo;
+        const uint64_t mid = lo + w * (uint64_t)(g + 1) / (G + 1);
+        const bool ok = lo < hi && pre(mid);
+        const int c = __popcll(__ballot(ok) & gmask);  // probes that hold: a prefix of the group's
+        if (lo < hi) {
+            const uint64_t m_lo = lo + w * (uint64_t)c / (G + 1);        // probe c - 1 (c > 0)
+            const uint64_t m_hi = lo + w * (uint64_t)(c + 1) / (G + 1);  // probe c (c < G)
+            if (c < G) hi = m_hi;
+            if (c > 0) lo = m_lo + 1;
+        }
+    }
+    return lo;
+}
+
 template <typename K, bool HV, int KW>
 __global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2, int slog2,
                                                      uint64_t Q, uint64_t ntg, uint64_t ntiles,
@@ -135,53 +162,41 @@ __global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in
         if (src >= 0 && (src >> 6) == i) mfound = x;
     }
     const bool to_end = p + 64 * NW >= gs_end;
-    if (lane >= KW) return;
-    const int j = lane;
+    // lanes [j*G, j*G + G) take run j: lane j's window result to its group
+    constexpr int G = 64 / KW;
+    const int j = lane / G;
+    const int has = __builtin_amdgcn_ds_bpermute(j << 2, src >= 0 ? 1 : 0);
+    const uint32_t mf = (uint32_t)__builtin_amdgcn_ds_bpermute(j << 2, (int)mfound);
+    K v;
+    int r;
+    uint64_t m;
+    Sample<K>::get(sm, p, v, r, m);  // the same in every lane
     const uint64_t lenj = kw_run_len(n, gbase, R, j);
+    // run j's items precede b: those <= v when j < r, those < v when j > r
+    const bool up = j < r;
+    const uint64_t ns_j = (lenj + (1ull << slog2) - 1) >> slog2;
+    const bool search = lenj > 0 && j != r;
+    // the number of run j's samples that precede b
+    const uint64_t sb = (gbase + (uint64_t)j * R) >> slog2;
+    const bool need = search && !has && !to_end;
+    uint64_t lo = group_search<G>(0, need ? ns_j : 0, lane, [&](uint64_t i) {
+        const K k = Sample<K>::key(s0, sb + i);
+        return up ? !(v < k) : k < v;
+    });
+    if (!need) lo = has ? (uint64_t)mf : ns_j;
+    // sample lo-1 precedes b and sample lo (if any) does not: b's place among
+    // the items between them
+    const uint64_t X = gbase + (uint64_t)j * R;
+    const bool win = search && lo > 0;
+    uint64_t a0 = win ? ((lo - 1) << slog2) + 1 : 0;
+    uint64_t b0 = win ? (lo << slog2 < lenj ? lo << slog2 : lenj) : 0;
+    const uint64_t a = group_search<G>(a0, b0, lane, [&](uint64_t i) {
+        const K k = GMem<K, HV>::load_key(in, X + i);
+        return up ? !(v < k) : k < v;
+    });
     uint64_t split = 0;
-    if (lenj > 0) {
-        K v;
-        int r;
-        uint64_t m;
-        Sample<K>::get(sm, p, v, r, m);
-        if (j == r) {
-            split = m << slog2;
-        } else {
-            // run j's items precede b: those <= v when j < r, those < v when j > r
-            const bool up = j < r;
-            const uint64_t ns_j = (lenj + (1ull << slog2) - 1) >> slog2;
-            uint64_t lo;
-            if (src >= 0) {
-                lo = mfound;
-            } else if (to_end) {
-                lo = ns_j;
-            } else {
-                const uint64_t sb = (gbase + (uint64_t)j * R) >> slog2;
-                uint64_t hi = ns_j;
-                lo = 0;
-                while (lo < hi) {
-                    const uint64_t mid = (lo + hi) >> 1;
-                    const K k = Sample<K>::key(s0, sb + mid);
-                    if (up ? !(v < k) : k < v) lo = mid + 1;
-                    else hi = mid;
-                }
-            }
-            if (lo > 0) {
-                // sample lo-1 precedes b and sample lo (if any) does not
-                const uint64_t X = gbase + (uint64_t)j * R;
-                uint64_t a = ((lo - 1) << slog2) + 1;
-                uint64_t b = lo << slog2;
-                b = b < lenj ? b : lenj;
-                while (a < b) {
-                    const uint64_t mid = (a + b) >> 1;
-                    const K k = GMem<K, HV>::load_key(in, X + mid);
-                    if (up ? !(v < k) : k < v) a = mid + 1;
-                    else b = mid;
-                }
-                split = a;
-            }
-        }
-    }
+    if (lenj > 0) split = j == r ? m << slog2 : (lo > 0 ? a : 0);
+    if (lane % G != 0) return;
     tab[t * KW + j] = split;
 }
 

@@ -416,3 +416,43 @@ def test_kway_kv_tile_model(kw, dist):
     allp = sorted(p for run in runs for p in run)
     assert [k for k, _ in out] == [k for k, _ in allp]
     assert sorted(out) == allp
+
+
+def group_search_model(lo, hi, G, pre):
+    """k_kway_bounds' group_search for one lane group: a (G+1)-ary search for the
+    first position in [lo, hi) where pre fails, one probe per lane per step."""
+    steps = 0
+    while lo < hi:
+        w = hi - lo
+        mids = [lo + w * (g + 1) // (G + 1) for g in range(G)]
+        oks = [pre(x) for x in mids]
+        c = sum(oks)
+        assert oks == [True] * c + [False] * (G - c)  # the holding probes are a prefix
+        m_lo = lo + w * c // (G + 1)
+        m_hi = lo + w * (c + 1) // (G + 1)
+        nlo, nhi = (m_lo + 1 if c > 0 else lo), (m_hi if c < G else hi)
+        assert nlo <= nhi and (nlo > lo or nhi < hi)  # progress every step
+        lo, hi = nlo, nhi
+        steps += 1
+    return lo, steps
+
+
+@pytest.mark.parametrize("G", [2, 4, 8, 16, 32])
+def test_group_search_matches_bisect(G):
+    import bisect
+    rng = np.random.default_rng(G)
+    for trial in range(400):
+        n = int(rng.integers(0, 5000)) if trial % 4 else int(rng.integers(0, 70))
+        keys = np.sort(rng.integers(0, max(1, n // 3 + 1), n))
+        v = int(rng.integers(-1, n // 3 + 2))
+        lo = int(rng.integers(0, n + 1))
+        hi = int(rng.integers(lo, n + 1))
+        for up in (True, False):
+            pre = (lambda i: not (v < keys[i])) if up else (lambda i: keys[i] < v)
+            got, steps = group_search_model(lo, hi, G, pre)
+            sub = keys[lo:hi]
+            ref = lo + (bisect.bisect_right(sub, v) if up else bisect.bisect_left(sub, v))
+            assert got == ref, (G, lo, hi, v, up)
+            # (G+1)-ary: about log_{G+1} of the range
+            if hi > lo:
+                assert steps <= int(np.ceil(np.log(hi - lo + 1) / np.log(G + 1))) + 2
