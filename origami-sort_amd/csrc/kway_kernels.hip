@@ -95,14 +95,21 @@ template <int G, typename F>
 __device__ __forceinline__ uint64_t group_search(uint64_t lo, uint64_t hi, int lane, F pre) {
     const int q = lane / G, g = lane % G;
     const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << (q * G);
+    // w * (G + 1) < 2^32 for every range below 2^26 (G < 64): then the
+    // probe offsets take 32-bit arithmetic (a wave-uniform choice)
+    const bool narrow = __ballot(hi - lo >= (1ull << 26)) == 0;
     while (__ballot(lo < hi) != 0) {
         const uint64_t w = hi - lo;
-        const uint64_t mid = lo + w * (uint64_t)(g + 1) / (G + 1);
+        const auto off = [&](int k) -> uint64_t {
+            return narrow ? (uint64_t)((uint32_t)w * (uint32_t)k / (uint32_t)(G + 1))
+                          : w * (uint64_t)k / (G + 1);
+        };
+        const uint64_t mid = lo + off(g + 1);
         const bool ok = lo < hi && pre(mid);
         const int c = __popcll(__ballot(ok) & gmask);  // probes that hold: a prefix of the group's
         if (lo < hi) {
-            const uint64_t m_lo = lo + w * (uint64_t)c / (G + 1);        // probe c - 1 (c > 0)
-            const uint64_t m_hi = lo + w * (uint64_t)(c + 1) / (G + 1);  // probe c (c < G)
+            const uint64_t m_lo = lo + off(c);      // probe c - 1 (c > 0)
+            const uint64_t m_hi = lo + off(c + 1);  // probe c (c < G)
             if (c < G) hi = m_hi;
             if (c > 0) lo = m_lo + 1;
         }
