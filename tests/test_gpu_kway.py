@@ -206,3 +206,20 @@ def test_kway_kv_shapes_and_max_keys(ctx, shape, sample):
         check_kv(ogs.KV_U64U64, gpu_sort(ctx, ogs.KV_U64U64, b), b)
     finally:
         restore(ctx, ogs.KV_U64U64)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.F32, ogs.U64, ogs.KV_U64U64])
+@pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.ZIPF, ogs.SORTED, ogs.REVERSE, ogs.DISTINCT16])
+def test_kway_default_tiles(ctx, kind, dist):
+    # the tile table's own plan (block shape, K-way shape, fan-in, sample
+    # spacing) on every distribution: sorted and reverse runs do not interleave,
+    # so the split kernel takes its search path, uniform keys its sample window
+    n = (3 << 20) + 777 if kind != ogs.KV_U64U64 else (3 << 19) + 555
+    say(f"case kway default tiles kind={ogs.KIND_NAMES[kind]} dist={dist} n={n}")
+    ctx.tune_load(ogs.DEFAULT_TILES)
+    assert ctx.tune_get(kind, "kway_fanin") > 0
+    a = ogs.generate(kind, dist, n, seed=77 + dist)
+    if kind == ogs.KV_U64U64:
+        check_kv(kind, gpu_sort(ctx, kind, a), a)
+    else:
+        check(kind, gpu_sort(ctx, kind, a), a)
