@@ -397,7 +397,23 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
         const uint64_t ns = (n + (1ull << sl[p]) - 1) >> sl[p];
         const void *sa = S0;
         void *sb[2] = {S1, S2};
-        for (int q = 0; q < lv[p]; q++) {
+        // 32-bit keys: a u64 sample is its own total order (key, run, index)
+        // and unique, so when a group's samples fill exactly one block-sort
+        // tile of the 64-bit class, one block sort replaces the merge passes
+        const BlockShapeFns *sbs = nullptr;
+        if (ki.kc == KC_32) {
+            const uint64_t gspan = 1ull << (rl - sl[p] + lv[p]);
+            int bcnt = 0;
+            const BlockShapeFns *bt = block_shapes(KC_64, &bcnt);
+            for (int j = 0; j < bcnt && !sbs; j++)
+                if (bt[j].tile() == gspan) sbs = &bt[j];
+        }
+        if (sbs) {
+            e = sbs->run(S0, S1, ns, 0, 0, s, nullptr, nullptr, nullptr);
+            if (e != hipSuccess) return hip_fail(e, "sample sort launch");
+            sa = S1;
+        }
+        for (int q = 0; q < (sbs ? 0 : lv[p]); q++) {
             const uint64_t Rs = 1ull << (rl - sl[p] + q);
             int best = -1;
             for (int j = 0; j < mcnt; j++)
