@@ -315,7 +315,7 @@ __device__ __forceinline__ int lds_corank_lift(const K *sk, int a_off, int la, i
         const K kb = lds_at_bytes(sk, pb - tb);
         ab = kb < ka ? ab : tb;
     }
-    return ab / W;
+    return (int)((uint32_t)ab / (uint32_t)W);  // ab >= 0
 }
 
 // Items of LDS guard space around a merge buffer: one 128-byte row before it
@@ -457,15 +457,18 @@ __device__ __forceinline__ int lds_corank_lift_rev(const K *sk, int a_off, int l
     const int lo = diag > lb ? diag - lb : 0;
     const int hi = diag < la ? diag : la;
     constexpr int W = (int)sizeof(K);
-    const int pa = (a_off - 1) * W, pb = (b_top - diag) * W;  // byte bases
-    int a = lo;
-    for (int s = top; s > 0; s >>= 1) {
-        const int t = min(a + s, hi);
-        const K ka = lds_at_bytes(sk, pa + t * W);
-        const K kb = lds_at_bytes(sk, pb + t * W);
-        a = kb < ka ? a : t;
+    // in bytes: A[t-1] at pa + t*W, B[diag-t] (stored descending) at pb + t*W;
+    // the answer and the step are kept in bytes too, so each step is one
+    // min, two adds, two swizzles, a compare and a select
+    const int pa = (a_off - 1) * W, pb = (b_top - diag) * W, hib = hi * W;
+    int ab = lo * W;
+    for (int s = top * W; s >= W; s >>= 1) {
+        const int tb = min(ab + s, hib);
+        const K ka = lds_at_bytes(sk, pa + tb);
+        const K kb = lds_at_bytes(sk, pb + tb);
+        ab = kb < ka ? ab : tb;
     }
-    return a;
+    return (int)((uint32_t)ab / (uint32_t)W);  // ab >= 0
 }
 
 // Serial merge on the bitonic layout from A's LDS index ia (moving up) and B's
