@@ -73,7 +73,7 @@ uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int
 }
 
 // ----------------------------------------------------------------------------
-// Tile splits: one wave per tile; tab[t * KW + j] = the split of run j at tile
+// Tile splits: one wave per tile (two for KW <= 8); tab[t * KW + j] = the split of run j at tile
 // t's first item, the boundary sample b = (v, r, m) at merged position p.  The
 // number of run j's samples that precede b is the index of run j's first sample
 // at or after p in the merged order (run j's samples appear there in index
@@ -117,30 +117,39 @@ __device__ __forceinline__ uint64_t group_search(uint64_t lo, uint64_t hi, int l
     return lo;
 }
 
+// Tiles per wave of the split kernel: two for fan-in 8 or less (32 lanes per
+// tile, at least 4 per run), else one.
+template <int KW> constexpr int kw_bounds_tpw() { return KW <= 8 ? 2 : 1; }
+
 template <typename K, bool HV, int KW>
 __global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2, int slog2,
                                                      uint64_t Q, uint64_t ntg, uint64_t ntiles,
                                                      const void *__restrict__ s0, const void *__restrict__ sm,
                                                      uint64_t *__restrict__ tab) {
-    constexpr int NW = KW * 8 / 64 > 1 ? KW * 8 / 64 : 1;  // window samples per lane
+    constexpr int TPW = kw_bounds_tpw<KW>();    // tiles per wave
+    constexpr int LPT = 64 / TPW;               // lanes per tile: [h * LPT, (h + 1) * LPT) take tile h
+    constexpr int NW = KW * 8 / LPT > 1 ? KW * 8 / LPT : 1;  // window samples per lane
+    constexpr int G = LPT / KW;                 // lanes per run
     const int lane = threadIdx.x & 63;
-    const uint64_t t = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (t >= ntiles) return;  // uniform in the wave
+    const int h = lane / LPT, hl = lane % LPT;
+    const uint64_t t0 = ((uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * TPW;
+    if (t0 >= ntiles) return;  // uniform in the wave
+    const uint64_t t = t0 + (uint64_t)h;
+    const bool valid = t < ntiles;
     const uint64_t R = 1ull << rlog2;
     const uint64_t g = t / ntg, tl = t % ntg;
     const uint64_t gbase = g * KW * R;
-    if (tl == 0) {
-        if (lane < KW) tab[t * KW + lane] = 0;
-        return;
-    }
+    const bool active = valid && tl > 0;  // the tile's splits need a search (tile 0 of a group starts at 0)
     const uint64_t gs0 = gbase >> slog2;  // the group's first merged sample
-    const uint64_t gs_end = gs0 + (gbase + KW * R >= n ? kw_last_samples(n, rlog2, slog2, KW) : (KW * R) >> slog2);
+    const uint64_t gs_end =
+        active ? gs0 + (gbase + KW * R >= n ? kw_last_samples(n, rlog2, slog2, KW) : (KW * R) >> slog2) : 0;
     const uint64_t p = gs0 + tl * Q;
     int rr[NW];
     uint32_t mm[NW];
+    K kk0 = 0;
 #pragma unroll
     for (int i = 0; i < NW; i++) {
-        const uint64_t idx = p + (uint64_t)(lane + 64 * i);
+        const uint64_t idx = p + (uint64_t)(hl + LPT * i);
         rr[i] = KW;  // no run
         mm[i] = 0;
         if (idx < gs_end) {
@@ -150,35 +159,45 @@ __global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in
             Sample<K>::get(sm, idx, v, r, m);
             rr[i] = r;
             mm[i] = (uint32_t)m;
+            if (i == 0) kk0 = v;
         }
     }
-    // lane j: the window position of run j's first sample (-1: none)
+    // lane h * LPT + j: the window position of run j's first sample in tile
+    // h's window (-1: none)
     int src = -1;
+    const uint64_t tmask = (LPT == 64 ? ~0ull : ((1ull << LPT) - 1)) << (h * LPT);
 #pragma unroll
     for (int i = 0; i < NW; i++) {
 #pragma unroll
         for (int j = 0; j < KW; j++) {
-            const uint64_t b = __ballot(rr[i] == j);
-            if (lane == j && src < 0 && b != 0) src = i * 64 + (__ffsll((unsigned long long)b) - 1);
+            const uint64_t b = __ballot(rr[i] == j) & tmask;
+            if (hl == j && src < 0 && b != 0) src = i * LPT + (__ffsll((unsigned long long)b) - 1 - h * LPT);
         }
     }
     uint32_t mfound = 0;
 #pragma unroll
     for (int i = 0; i < NW; i++) {
-        const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute((src >= 0 ? (src & 63) : 0) << 2, (int)mm[i]);
-        if (src >= 0 && (src >> 6) == i) mfound = x;
+        const int sl = h * LPT + (src >= 0 ? src % LPT : 0);
+        const uint32_t x = (uint32_t)__builtin_amdgcn_ds_bpermute(sl << 2, (int)mm[i]);
+        if (src >= 0 && src / LPT == i) mfound = x;
     }
-    const bool to_end = p + 64 * NW >= gs_end;
-    // lanes [j*G, j*G + G) take run j: lane j's window result to its group
-    constexpr int G = 64 / KW;
-    const int j = lane / G;
-    const int has = __builtin_amdgcn_ds_bpermute(j << 2, src >= 0 ? 1 : 0);
-    const uint32_t mf = (uint32_t)__builtin_amdgcn_ds_bpermute(j << 2, (int)mfound);
+    const bool to_end = p + (uint64_t)(LPT * NW) >= gs_end;
+    // lanes [h * LPT + j * G, ... + G) take run j of tile h: the window result
+    // of lane h * LPT + j to its group, and the boundary sample (v, r, m),
+    // window item 0 of the tile, from lane h * LPT
+    const int j = hl / G;
+    const int has = __builtin_amdgcn_ds_bpermute((h * LPT + j) << 2, src >= 0 ? 1 : 0);
+    const uint32_t mf = (uint32_t)__builtin_amdgcn_ds_bpermute((h * LPT + j) << 2, (int)mfound);
     K v;
-    int r;
-    uint64_t m;
-    Sample<K>::get(sm, p, v, r, m);  // the same in every lane
-    const uint64_t lenj = kw_run_len(n, gbase, R, j);
+    if constexpr (sizeof(K) == 4) {
+        v = (K)__builtin_amdgcn_ds_bpermute((h * LPT) << 2, (int)kk0);
+    } else {
+        v = ((K)(uint32_t)__builtin_amdgcn_ds_bpermute((h * LPT) << 2, (int)(kk0 >> 32)) << 32) |
+            (uint32_t)__builtin_amdgcn_ds_bpermute((h * LPT) << 2, (int)(uint32_t)kk0);
+    }
+    const int r = __builtin_amdgcn_ds_bpermute((h * LPT) << 2, rr[0]);
+    const uint64_t m = (uint32_t)__builtin_amdgcn_ds_bpermute((h * LPT) << 2, (int)mm[0]);
+    const uint64_t lenj = active ? kw_run_len(n, gbase, R, j) : 0;
     // run j's items precede b: those <= v when j < r, those < v when j > r
     const bool up = j < r;
     const uint64_t ns_j = (lenj + (1ull << slog2) - 1) >> slog2;
@@ -203,7 +222,7 @@ __global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in
     });
     uint64_t split = 0;
     if (lenj > 0) split = j == r ? m << slog2 : (lo > 0 ? a : 0);
-    if (lane % G != 0) return;
+    if (!valid || lane % G != 0) return;
     tab[t * KW + j] = split;
 }
 
@@ -547,7 +566,8 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     const uint64_t ntg = kw_group_tiles(GR >> slog2, Q);
     const uint64_t nt_last = kw_group_tiles(kw_last_samples(n, rlog2, slog2, KW), Q);
     const uint64_t ntiles = (n + GR - 1) / GR * ntg - ntg + nt_last;
-    hipLaunchKernelGGL((k_kway_bounds<K, HV, KW>), dim3((unsigned)((ntiles + 3) / 4)), dim3(256), 0, s, in, n,
+    const uint64_t tpb = 4 * (uint64_t)kw_bounds_tpw<KW>();  // tiles per 256-thread block
+    hipLaunchKernelGGL((k_kway_bounds<K, HV, KW>), dim3((unsigned)((ntiles + tpb - 1) / tpb)), dim3(256), 0, s, in, n,
                        rlog2, slog2, Q, ntg, ntiles, s0, sm, tab);
     if (mid) (void)hipEventRecord(mid, s);
     const SampleSpec none = {nullptr, 0, 0, 0};
