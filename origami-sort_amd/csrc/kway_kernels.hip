@@ -73,16 +73,17 @@ uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int
 }
 
 // ----------------------------------------------------------------------------
-// Tile splits: one wave per tile (two for KW <= 8); tab[t * KW + j] = the split of run j at tile
-// t's first item, the boundary sample b = (v, r, m) at merged position p.  The
-// number of run j's samples that precede b is the index of run j's first sample
-// at or after p in the merged order (run j's samples appear there in index
-// order), so the wave reads the 8 * KW merged samples from p on, and lane j
-// takes run j's first one by ballot.  A run with no sample in that window has
-// all its samples before p when the window reaches the group's end, else the
-// count comes from a search over run j's own samples.  A second search then
-// places b among the s items that separate two samples of run j.  Each search
-// is run by the G = 64 / KW lanes of run j together (group_search).
+// Tile splits: one wave per tile (two tiles of 32 lanes for KW <= 8);
+// tab[t * KW + j] = the split of run j at tile t's first item, the boundary
+// sample b = (v, r, m) at merged position p.  The number of run j's samples
+// that precede b is the index of run j's first sample at or after p in the
+// merged order (run j's samples appear there in index order), so the tile's
+// lanes read the 8 * KW merged samples from p on, and its lane j takes run j's
+// first one by ballot.  A run with no sample in that window has all its
+// samples before p when the window reaches the group's end, else the count
+// comes from a search over run j's own samples.  A second search then places b
+// among the s items that separate two samples of run j.  Each search is run by
+// the G = (lanes per tile) / KW lanes of run j together (group_search).
 // ----------------------------------------------------------------------------
 
 // The first position in [lo, hi) whose item does not satisfy pre() (hi if all
