@@ -8,12 +8,14 @@ copy runs inside the timed region).  Prints one JSON line:
   value        whole-job G keys/s over the K timed steps
   roofline     the dominant kernel (by summed time) vs the 8 TB/s HBM peak,
                algorithmic bytes per launch (2 * n * w) / its mean launch time,
-               timed with HIP events recorded on the launching stream during
-               the last timed step (events between every kernel of every step
-               would add a few microseconds per kernel to the whole-job time)
+               timed with HIP events recorded on the launching stream over
+               --profile-steps sorts run after the timed region (events between
+               every kernel add a few microseconds per kernel, so the timed
+               steps run without them)
   pass_roofline the pass-count roofline of the whole sort: P * 2 * n * w / t
-  cpu_baseline the oracle (a C restatement of Origami's mergesort) on a bounded
-               sample of the same input on this host's cores
+  cpu_baseline the oracle (a scalar C restatement of Origami's mergesort, the
+               reference's REG_TYPE 0 build) on the same full input, on a
+               power-of-two number of this host's cores (at most 16), min_k = 4
 """
 import argparse
 import json
@@ -47,44 +49,97 @@ def traffic_from_profiles(kernel, workload):
         return None
 
 
-def valu_from_profiles(kernel, workload, n):
-    """The dominant kernel's VALU issue figures from the committed rocprofv3 SQ
-    counters (profiles/pmc_traffic.json), when they are of this workload: its
-    instructions per key and its issue rate against the one-instruction-per-CU-
-    cycle ceiling of a wave64 VALU op on four 16-lane SIMDs."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+def host_cpu():
+    """CPU model, physical cores and logical CPUs of this host (/proc/cpuinfo)."""
+    model, phys = "unknown", set()
+    logical = os.cpu_count() or 1
+    cur = {}
     try:
-        with open(path) as f:
-            d = json.load(f)
-        if d.get("bench_workload") != workload:
-            return None
-        v = d.get(kernel, {}).get("valu")
-        if not v:
-            return None
-        # SQ_INSTS_VALU counts wave64 instructions: 64 lane operations each
-        return {"insts_per_key": round(v["insts_per_launch"] * 64 / n, 1),
-                "per_cu_cycle": v["per_cu_cycle"], "ceiling_per_cu_cycle": 1.0,
-                "source": "profiles/pmc_traffic.json (rocprofv3 SQ_INSTS_VALU, SQ_BUSY_CYCLES)"}
-    except (OSError, ValueError, KeyError):
-        return None
+        with open("/proc/cpuinfo") as f:
+            for line in f.read().splitlines() + [""]:
+                if not line.strip():
+                    if "core id" in cur:
+                        phys.add((cur.get("physical id", "0"), cur["core id"]))
+                    cur = {}
+                    continue
+                k, _, v = line.partition(":")
+                cur[k.strip()] = v.strip()
+                if k.strip() == "model name" and model == "unknown":
+                    model = v.strip()
+    except OSError:
+        pass
+    return model, (len(phys) or logical), logical
 
 
-def cpu_baseline(kind_oracle, host_keys, sample_n, threads):
-    import numpy as np
-
+def cpu_baseline(kind_oracle, host_keys, max_threads=16):
+    """The oracle's multithreaded driver (sort_multi_thread restated,
+    Origami/sorter.h:4194-4287, scalar compare-exchanges as in the reference's
+    REG_TYPE 0 build) on the full input, timed like Origami/bench_sort.cpp:94-129:
+    the sort call only, 3 repeats, mean and median; n_threads = the largest
+    power of two <= the physical cores, at most max_threads (the host share of
+    one GPU), min_k = 4 (Origami/README.md:557)."""
     import oracle
-    sample = np.ascontiguousarray(host_keys[:sample_n])
+    model, phys, logical = host_cpu()
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else logical
+    threads = 1
+    while threads * 2 <= min(phys, avail, max_threads, 64):
+        threads *= 2
+    n = host_keys.size
     times = []
     for _ in range(3):
         t0 = time.perf_counter()
-        out = oracle.sort(kind_oracle, sample, threads=threads, min_k=2)
+        out = oracle.sort(kind_oracle, host_keys, threads=threads, min_k=4)
         times.append(time.perf_counter() - t0)
-    keys = out["key"] if out.dtype.names else out
-    assert bool((keys[1:] >= keys[:-1]).all())
-    t = sorted(times)[1]
-    return {"value": round(sample_n / t / 1e9, 6), "unit": "G keys/s", "cores": threads, "kind": "port",
-            "sample": f"first 2^{sample_n.bit_length() - 1} keys of the same input, median of 3, "
-                      f"oracle/origami_oracle.c sort_multi_thread restatement, {threads} threads, min_k=2"}
+    del out
+    mean, med = sum(times) / 3, sorted(times)[1]
+    return {"value": round(n / med / 1e9, 6), "unit": "G keys/s", "cores": threads, "kind": "port",
+            "mean_value": round(n / mean / 1e9, 6), "cpu_model": model, "physical_cores": phys,
+            "logical_cpus": logical,
+            "sample": f"the full input (2^{n.bit_length() - 1} keys), median of 3 sort calls; "
+                      f"oracle/origami_oracle.c sort_multi_thread restatement, scalar restatement (REG_TYPE 0), "
+                      f"{threads} threads, min_k=4; {model}, {phys} physical cores"}
+
+
+def result_checks(kind, d_in, d_out, n):
+    """Size-independent checks of a sort's device output against its input:
+    (sorted in the key order, the same multiset of items).  The multiset check
+    compares the wrapping sum, the xor and the wrapping sum of squares of the
+    items as 64-bit words (4-byte kinds: of the keys widened to 64 bits)."""
+    import torch
+
+    import ogs
+    w = ogs.WIDTH[kind]
+
+    def words(t):
+        if w == 4:
+            return t[: n * 4].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+        return t[: n * w].view(torch.int64)
+
+    def xor_all(t):
+        t = t.clone()
+        m = t.numel()
+        while m > 1:  # halving xor-reduction
+            h = m // 2
+            t[:h] ^= t[h:2 * h]
+            if m & 1:
+                t[0] ^= t[m - 1]
+            m = h
+        return int(t[0]) if t.numel() else 0
+
+    a, b = words(d_in), words(d_out)
+    same = (int(a.sum()) == int(b.sum()) and xor_all(a) == xor_all(b)
+            and int((a * a).sum()) == int((b * b).sum()))
+    # each item's key in the sort order as an order-preserving int64
+    if kind == ogs.F32:
+        k = torch.where(b >= 0x80000000, 0xFFFFFFFF - b, b + 0x80000000)
+    elif w == 4:
+        k = b
+    else:
+        k = (b[0::2] if w == 16 else b) ^ torch.tensor(-(1 << 63), dtype=torch.int64, device=b.device)
+    ok_sorted = bool((k[1:] >= k[:-1]).all())
+    del a, b, k
+    torch.cuda.empty_cache()
+    return ok_sorted, same
 
 
 def bench_batched(ctx, total_log2, max_seg_log2, seed=4, nseg_fixed=0):
@@ -318,6 +373,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="sorts run with per-kernel HIP events after the timed region (0: --steps)")
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--kind", default="u32", choices=["u32", "f32", "u64", "kv_u64u64"])
     ap.add_argument("--dist", default=None,
@@ -367,18 +424,17 @@ def main():
     for t in args.tune:
         pname, pval = t.split("=")
         ctx.tune_set(kind, pname, int(pval))
-    sample_n = min(n, 1 << 26)
     if dist < ogs.SAME:
         log(f"bench: generating {n} {args.kind} keys, {dname} (seed 42)")
         host = ogs.generate(kind, dist, n, seed=42)
         d_in = ogs.to_device(host)
     else:
-        # counter-based family, written on the device; the host keeps only the
-        # CPU baseline's sample
+        # counter-based family, written on the device (the host copy is the
+        # CPU baseline's input)
         log(f"bench: generating {n} {args.kind} keys on the device, {dname} (seed 42)")
         d_in = torch.empty(n * w, dtype=torch.uint8, device="cuda")
         ctx.generate_ptr(kind, dist, 42, n, d_in.data_ptr())
-        host = d_in[: sample_n * w].cpu().numpy().view(ogs.NP_DTYPE[kind])
+        host = d_in.cpu().numpy().view(ogs.NP_DTYPE[kind])
     d_a = torch.empty_like(d_in)
     d_b = torch.empty_like(d_in)
     torch.cuda.synchronize()
@@ -391,32 +447,33 @@ def main():
     for _ in range(args.warmup):
         res = step()
     torch.cuda.synchronize()
-    # size-independent checks of the result: sorted (in the key order) and the
-    # same multiset (sum and xor of the raw bytes as 64-bit words)
+    # size-independent checks of the result: sorted in the key order, and the
+    # same multiset of items (wrapping sum, xor and sum of squares of the raw
+    # bytes as 64-bit words; for 4-byte keys, of the keys)
     out_t = d_a if res == d_a.data_ptr() else d_b
-    if args.kind == "u32":
-        o = out_t.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-        ok_sorted = bool((o[1:] >= o[:-1]).all())
-        ok_sum = int(o.sum()) == int((d_in.view(torch.int32).to(torch.int64) & 0xFFFFFFFF).sum())
-        log(f"bench: result sorted={ok_sorted} multiset-sum={ok_sum}")
-        assert ok_sorted and ok_sum
-        del o
+    ok_sorted, ok_multiset = result_checks(kind, d_in, out_t, n)
+    log(f"bench: result sorted={ok_sorted} multiset={ok_multiset}")
+    assert ok_sorted and ok_multiset
 
-    ctx.profile_reset()
     torch.cuda.synchronize()
     log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
     for i in range(args.steps):
-        # per-kernel HIP events on the last timed step only: each event adds a
-        # few microseconds between dependent kernels
-        if i == args.steps - 1:
-            ctx.profile(True)
         step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    ctx.profile(False)
     ms_step = (t1 - t0) * 1e3 / args.steps
     value = n / (ms_step / 1e3) / 1e9
+
+    # per-kernel times: HIP events on the launch stream around every kernel of
+    # prof_steps further sorts (outside the timed region)
+    prof_steps = args.profile_steps or args.steps
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(prof_steps):
+        step()
+    torch.cuda.synchronize()
+    ctx.profile(False)
 
     kernels = {}
     for kid, name in ((0, "block_sort"), (1, "merge_partition"), (2, "merge_pass"), (3, "merge4_pass"),
@@ -450,17 +507,15 @@ def main():
         "config": {"workload": workload, "n": n, "item_bytes": w},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(dom, workload)},
+                     "traffic": traffic_from_profiles(dom, workload),
+                     "timing": f"HIP events on the launch stream, mean over {dk['launches']} launches in "
+                               f"{prof_steps} sorts after the timed region"},
         "pass_roofline": {"P": P, "bytes_per_key": P * 2 * w, "achieved_GBs": round(pass_gbs, 2),
                           "frac": round(pass_gbs / HBM_PEAK_GBS, 4),
                           "single_pass_GBs": round(2.0 * n * w / (ms_step / 1e3) / 1e9, 2)},
         "kernels": {k: {kk: (round(vv, 5) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                     for k, v in kernels.items()},
     }
-    valu = valu_from_profiles(dom, workload, n)
-    if valu:
-        # the dominant kernel's issue bound (it runs below the HBM roofline)
-        line["valu"] = dict(kernel=dom, **valu)
     if args.copy_calib:
         for _ in range(3):
             d_b.copy_(d_in)
@@ -477,11 +532,10 @@ def main():
         line["serving"] = bench_serving(ctx, args.serving_requests, args.serving_load, args.serving_streams,
                                         coalesce=args.serving_coalesce)
     if not args.no_cpu_baseline:
-        threads = max(1, min(16, os.cpu_count() or 1))
         import oracle
         okind = {"u32": oracle.U32, "f32": oracle.F32, "u64": oracle.U64, "kv_u64u64": oracle.KV_U64U64}[args.kind]
-        log(f"bench: cpu baseline on {threads} threads")
-        line["cpu_baseline"] = cpu_baseline(okind, host, sample_n, threads)
+        log("bench: cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(okind, host)
     print(json.dumps(line), flush=True)
 
 

@@ -10,21 +10,27 @@
  *                                                                  Origami/sorter.h:4194-4195
  * with the item type fixed at compile time (Origami/config.h:23-24): uint32,
  * signed int64, or a packed 16-byte {int64 key, int64 value}
- * (Origami/commons.h:51-72).  This ABI takes the same shapes at run time and
- * adds the types the reference lacks (int32, uint64, float32, unsigned KV).
+ * (Origami/commons.h:51-72); the reference also declares the packed 8-byte
+ * KeyValue<ui, ui> (Origami/commons.h:71).  This ABI takes the same shapes at
+ * run time and adds the types the reference lacks (int32, uint64, float32,
+ * unsigned 16-byte KV).
  * include/origami_gpu_compat.h re-creates the two template entry points on
  * top of it, so reference host code recompiles unchanged.
  *
  * Conventions
  *  - Device pointers unless a name says _host.  Buffers hold n items; items
- *    are 4 bytes (U32/I32/F32), 8 bytes (U64/I64) or 16 bytes (KV_*: key at
- *    offset 0, value at offset 8, like KeyValue<i64,i64>).
+ *    are 4 bytes (U32/I32/F32), 8 bytes (U64/I64, and KV_U32U32: key at
+ *    offset 0, value at offset 4, like KeyValue<ui,ui>) or 16 bytes
+ *    (KV_U64U64/KV_I64I64: key at offset 0, value at offset 8, like
+ *    KeyValue<i64,i64>).
  *  - Sorting is ascending.  The result lands in *sorted, which is d_data or
  *    d_scratch (the other one is clobbered), as the reference returns d or
  *    tmp (Origami/merger.h:4225, Origami/sorter.h:3909-3914).
  *  - Key-only results are the unique sorted permutation.  KV sorts are not
  *    stable (neither is the reference): keys come out sorted and each key
- *    keeps its multiset of values.
+ *    keeps its multiset of values.  KV_U32U32 sorts each record as the
+ *    64-bit number key * 2^32 + value, so equal keys come out with their
+ *    values ascending (one of the orders the reference may produce).
  *  - F32 keys sort in IEEE-754 totalOrder (-NaN < -inf < ... < -0 < +0 < ... <
  *    +inf < +NaN).  I32/I64/KV_I64I64 keys are signed; U32/U64/KV_U64U64 are
  *    unsigned.
@@ -52,7 +58,8 @@ typedef enum {
     OGS_I64 = 3,        /* int64   (Origami KEY_TYPE 1) */
     OGS_F32 = 4,        /* float32 (no reference counterpart) */
     OGS_KV_U64U64 = 5,  /* {uint64 key, uint64 value} (unsigned form of KEY_TYPE 2) */
-    OGS_KV_I64I64 = 6   /* {int64 key, int64 value}   (Origami KEY_TYPE 2) */
+    OGS_KV_I64I64 = 6,  /* {int64 key, int64 value}   (Origami KEY_TYPE 2) */
+    OGS_KV_U32U32 = 7   /* {uint32 key, uint32 value} (KeyValue<ui, ui>, Origami/commons.h:71) */
 } ogs_key_t;
 
 typedef enum {
@@ -70,6 +77,15 @@ typedef struct ogs_ctx ogs_ctx;
  * Replaces the reference's process globals (Origami/sorter.h:3941-3959). */
 int ogs_ctx_create(ogs_ctx **ctx, int device);
 int ogs_ctx_destroy(ogs_ctx *ctx);
+
+/* Device error word of a context: waits for the device, returns the bits set
+ * since the last call in *flags and clears them.  Bit 0: a K-way merge tile
+ * found its split table inconsistent, bit 1: the same for a 4-way merge tile;
+ * such a tile is left unmerged, so its output range is not the sort.  Neither
+ * happens with the tables the library's split kernels write; stream-ordered
+ * calls cannot report it in their return code, ogs_sort_host does.  No
+ * reference counterpart (its ReportError exits, Origami/commons.h:104). */
+int ogs_device_errors(ogs_ctx *ctx, uint32_t *flags);
 
 /* Bytes per item of a key type (0 for an unknown type). */
 size_t ogs_item_size(ogs_key_t kt);
@@ -148,7 +164,9 @@ int ogs_plan_destroy(ogs_plan *plan);
 
 /* Host-memory convenience: copies h_data (n items) to the device, sorts, and
  * copies the result back into h_data; synchronous.  This is what
- * origami_gpu_compat.h calls for the reference's host-pointer signatures. */
+ * origami_gpu_compat.h calls for the reference's host-pointer signatures.
+ * Returns OGS_ERR_STATE when the device error word (ogs_device_errors) was
+ * set during the call. */
 int ogs_sort_host(ogs_ctx *ctx, ogs_key_t kt, void *h_data, uint64_t n, uint64_t chunk);
 
 /* Analytic HBM pass count P of a plain sort of n items of type kt with the
@@ -207,10 +225,15 @@ int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value);
 int ogs_tune_get(ogs_ctx *ctx, ogs_key_t kt, const char *param, int *value);
 
 /* Applies a tile-table file: lines "kind param value" (kind u32, i32, f32,
- * u64, i64, kv_u64u64 or kv_i64i64), '#' starts a comment.  ogs_ctx_create
- * applies $OGS_TILES, or else ogs_tiles.txt next to libogs.so, when present;
- * the autotuner (origami-sort_amd/autotune.py) writes that file. */
+ * u64, i64, kv_u64u64, kv_i64i64 or kv_u32u32), '#' starts a comment; all
+ * lines or none (a bad line leaves the table unchanged and names itself in
+ * ogs_last_error).  A context starts from the compiled-in table, measured on
+ * MI355X; ogs_ctx_create then applies the file named by $OGS_TILES, when set,
+ * and fails if it does not apply.  The autotuner
+ * (origami-sort_amd/autotune.py) writes such files. */
 int ogs_tune_load(ogs_ctx *ctx, const char *path);
+/* Restores every key class's compiled-in tile table. */
+int ogs_tune_reset(ogs_ctx *ctx);
 
 /* Compiled kernel shapes of a key type: merge = 0 lists block-sort shapes
  * (workgroup size, items per lane, in-wave bitonic levels), merge = 1 merge

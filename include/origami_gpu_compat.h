@@ -6,7 +6,9 @@
  *       Origami/sorter.h:3851-3852
  *   origami_sorter::sort_multi_thread<Item, Reg>(d, out, n, n_threads, n_cores, min_k, kway_buf)
  *       Origami/sorter.h:4194-4195
- * on host arrays of ui (uint32), i64 or KeyValue<i64, i64> (Origami/commons.h:47-72).
+ * on host arrays of ui (uint32), i64, KeyValue<i64, i64> or KeyValue<ui, ui>
+ * (Origami/commons.h:47-72), and also of int32, float, ui64 and
+ * KeyValue<ui64, ui64>; any other item type fails to compile.
  * Including this header instead of sorter.h keeps those calls compiling; they
  * run on the GPU through libogs.so (include/ogs.h).  The SIMD register type,
  * thread counts, min_k and scratch buffers only tune the CPU algorithm and are
@@ -34,8 +36,9 @@ typedef int64_t i64;
 typedef uint64_t ui64;
 typedef uint32_t ui;
 
-/* Packed 16-byte {key, value} record, laid out like the reference's
- * KeyValue (Origami/commons.h:51-72); comparisons look at the key only. */
+/* Packed {key, value} record, laid out like the reference's KeyValue
+ * (Origami/commons.h:51-72): 16 bytes for 64-bit keys, 8 for KeyValue<ui, ui>;
+ * comparisons look at the key only. */
 #pragma pack(push, 1)
 template <typename Keytype, typename Valuetype>
 struct KeyValue {
@@ -65,15 +68,28 @@ inline ogs_ctx *context() {
     return ctx;
 }
 
+template <typename T> struct always_false : std::false_type {};
+
+// The ABI kind of an item type.  Only the types below map; anything else
+// (unsigned long long, double, KeyValue<i32, i32>, ...) stops the build here,
+// so no item is ever read with another type's width.
 template <typename Item> constexpr ogs_key_t key_type() {
-    static_assert(sizeof(Item) == 4 || sizeof(Item) == 8 || sizeof(Item) == 16, "unsupported item type");
     if constexpr (std::is_same<Item, uint32_t>::value) return OGS_U32;
     else if constexpr (std::is_same<Item, int32_t>::value) return OGS_I32;
     else if constexpr (std::is_same<Item, float>::value) return OGS_F32;
     else if constexpr (std::is_same<Item, uint64_t>::value) return OGS_U64;
     else if constexpr (std::is_same<Item, int64_t>::value) return OGS_I64;
-    else if constexpr (sizeof(Item) == 16 && std::is_signed<decltype(Item::key)>::value) return OGS_KV_I64I64;
-    else return OGS_KV_U64U64;
+    else if constexpr (std::is_same<Item, KeyValue<int64_t, int64_t>>::value) return OGS_KV_I64I64;
+    else if constexpr (std::is_same<Item, KeyValue<uint64_t, uint64_t>>::value) return OGS_KV_U64U64;
+    else if constexpr (std::is_same<Item, KeyValue<uint32_t, uint32_t>>::value) {
+        static_assert(sizeof(Item) == 8, "KeyValue<ui, ui> must be the packed 8-byte record");
+        return OGS_KV_U32U32;
+    } else {
+        static_assert(always_false<Item>::value,
+                      "origami_gpu_compat: unsupported item type (ui, i32, float, ui64, i64, KeyValue<i64, i64>, "
+                      "KeyValue<ui64, ui64> or KeyValue<ui, ui>)");
+        return OGS_U32;
+    }
 }
 
 template <typename Item> Item *run(Item *d, uint64_t n, uint64_t chunk) {

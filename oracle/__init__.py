@@ -16,14 +16,15 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 # key kinds: the same numbering as ogs_key_t in include/ogs.h
-U32, I32, U64, I64, F32, KV_U64U64, KV_I64I64 = range(7)
+U32, I32, U64, I64, F32, KV_U64U64, KV_I64I64, KV_U32U32 = range(8)
 KINDS = {"u32": U32, "i32": I32, "u64": U64, "i64": I64, "f32": F32,
-         "kv_u64u64": KV_U64U64, "kv_i64i64": KV_I64I64}
+         "kv_u64u64": KV_U64U64, "kv_i64i64": KV_I64I64, "kv_u32u32": KV_U32U32}
 KV_DTYPE = np.dtype([("key", "<u8"), ("value", "<u8")])
 KV_I_DTYPE = np.dtype([("key", "<i8"), ("value", "<i8")])
+KV32_DTYPE = np.dtype([("key", "<u4"), ("value", "<u4")])
 DTYPES = {U32: np.dtype("<u4"), I32: np.dtype("<i4"), U64: np.dtype("<u8"),
           I64: np.dtype("<i8"), F32: np.dtype("<f4"), KV_U64U64: KV_DTYPE,
-          KV_I64I64: KV_I_DTYPE}
+          KV_I64I64: KV_I_DTYPE, KV_U32U32: KV32_DTYPE}
 # distributions of orc_generate
 UNIFORM, ZIPF, SORTED, REVERSE, DISTINCT16, NORMAL = range(6)
 SAME, ALMOST_SORTED, PARETO, PARETO_BURSTS, PARETO_SHUFFLED, FIBONACCI = range(6, 12)

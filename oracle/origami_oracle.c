@@ -34,9 +34,12 @@
 #include <omp.h>
 
 enum { ORC_U32 = 0, ORC_I32 = 1, ORC_U64 = 2, ORC_I64 = 3, ORC_F32 = 4,
-       ORC_KV_U64U64 = 5, ORC_KV_I64I64 = 6 };
+       ORC_KV_U64U64 = 5, ORC_KV_I64I64 = 6, ORC_KV_U32U32 = 7 };
 
 typedef struct { uint64_t k, v; } kv_t;
+/* the reference's 8-byte KeyValue<ui, ui> (Origami/commons.h:51-71): packed
+ * {u32 key, u32 value}, compared on the key only like every KeyValue */
+typedef struct { uint32_t k, v; } kv32_t;
 
 /* ------------------------------------------------------------------------ */
 /* Seeded generators.  datagen::Writer (writer.h:47-242) draws with a        */
@@ -347,12 +350,13 @@ static T *sort_##S(T *d, T *tmp, uint64_t n, uint64_t sort_n, int p1n, uint64_t 
 ORC_GEN(uint32_t, u32, LT_PLAIN)
 ORC_GEN(uint64_t, u64, LT_PLAIN)
 ORC_GEN(kv_t, kv, LT_KV)
+ORC_GEN(kv32_t, kv32, LT_KV)
 
 static int width_of(int kt)
 {
     switch (kt) {
     case ORC_U32: case ORC_I32: case ORC_F32: return 4;
-    case ORC_U64: case ORC_I64: return 8;
+    case ORC_U64: case ORC_I64: case ORC_KV_U32U32: return 8;
     case ORC_KV_U64U64: case ORC_KV_I64I64: return 16;
     default: return 0;
     }
@@ -380,6 +384,8 @@ int orc_sort(int kt, void *data, void *tmp, uint64_t n, uint64_t sort_n, int thr
     void *res;
     uint64_t p2n = (512u * 1024u) / (uint64_t)w;
     if (w == 4) res = sort_u32((uint32_t *)data, (uint32_t *)tmp, n, sort_n, 8, p2n, 4, threads, min_k);
+    else if (kt == ORC_KV_U32U32)  /* a KV type: P1_N 16, one merge register (config.h:82-95) */
+        res = sort_kv32((kv32_t *)data, (kv32_t *)tmp, n, sort_n, 16, p2n, 1, threads, min_k);
     else if (w == 8) res = sort_u64((uint64_t *)data, (uint64_t *)tmp, n, sort_n, 8, p2n, 4, threads, min_k);
     else res = sort_kv((kv_t *)data, (kv_t *)tmp, n, sort_n, 16, p2n, 1, threads, min_k);
     if (res != data) memcpy(data, res, n * (uint64_t)w);
@@ -434,6 +440,7 @@ int orc_merge(int kt, int nreg, const void *A, uint64_t la, const void *B, uint6
 {
     int w = width_of(kt);
     if (w == 4) mergebl_u32((const uint32_t *)A, la, (const uint32_t *)B, lb, (uint32_t *)C, nreg);
+    else if (kt == ORC_KV_U32U32) mergebl_kv32((const kv32_t *)A, la, (const kv32_t *)B, lb, (kv32_t *)C, 1);
     else if (w == 8) mergebl_u64((const uint64_t *)A, la, (const uint64_t *)B, lb, (uint64_t *)C, nreg);
     else if (w == 16) mergebl_kv((const kv_t *)A, la, (const kv_t *)B, lb, (kv_t *)C, 1);
     else return -1;
@@ -450,6 +457,7 @@ int orc_kway_merge(int kt, const void *base, const uint64_t *lens, int K, void *
     for (int k = 0; k < K; k++) { rp[k] = (const char *)base + o * (uint64_t)w; o += lens[k]; }
     if (bufn < 1) bufn = 1;
     if (w == 4) kway_u32((const uint32_t *const *)rp, lens, K, (uint32_t *)out, bufn);
+    else if (kt == ORC_KV_U32U32) kway_kv32((const kv32_t *const *)rp, lens, K, (kv32_t *)out, bufn);
     else if (w == 8) kway_u64((const uint64_t *const *)rp, lens, K, (uint64_t *)out, bufn);
     else kway_kv((const kv_t *const *)rp, lens, K, (kv_t *)out, bufn);
     free(rp);
@@ -466,6 +474,7 @@ int orc_partition(int kt, const void *base, const uint64_t *lens, int K, uint64_
     for (int k = 0; k < K; k++) { rp[k] = (const char *)base + o * (uint64_t)w; o += lens[k]; }
     if (rank > o) { free(rp); return -1; }
     if (w == 4) partition_u32((const uint32_t *const *)rp, lens, K, rank, splits);
+    else if (kt == ORC_KV_U32U32) partition_kv32((const kv32_t *const *)rp, lens, K, rank, splits);
     else if (w == 8) partition_u64((const uint64_t *const *)rp, lens, K, rank, splits);
     else partition_kv((const kv_t *const *)rp, lens, K, rank, splits);
     free(rp);
@@ -625,6 +634,16 @@ int orc_generate(int kt, int dist, uint64_t seed, uint64_t n, void *out)
 {
     int w = width_of(kt);
     if (!w) return -1;
+    if (kt == ORC_KV_U32U32) {
+        /* {u32 key, u32 value}: the u32 keys of the same distribution, value =
+         * original index */
+        uint32_t *k = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+        if (!k) return -1;
+        int rc = orc_generate(ORC_U32, dist, seed, n, k);
+        for (uint64_t i = 0; rc == 0 && i < n; i++) { ((kv32_t *)out)[i].k = k[i]; ((kv32_t *)out)[i].v = (uint32_t)i; }
+        free(k);
+        return rc;
+    }
     if ((dist & 0x100) || (dist >= 6 && dist < 12)) {
         int base = dist & ~0x100;
         if (base < 0 || base >= 12) return -1;

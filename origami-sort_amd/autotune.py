@@ -8,8 +8,8 @@ For each key class (u32 / u64 / KV) it times, on the GPU:
   - segmented sorts of a log-uniform batch: the large-class block and merge
     shapes, then the small class (one workgroup per segment), then the tiny
     class (one wavefront-sized workgroup per segment) and its cut-off;
-and writes the winners as a tile-table file (ogs_tune_load format) next to
-libogs.so, where every new context picks it up.
+and writes the winners as a tile-table file (ogs_tune_load format); a context
+applies it when $OGS_TILES names it (the compiled-in table is the default).
 
   python origami-sort_amd/autotune.py [--quick] [--out PATH]
 """
@@ -210,9 +210,10 @@ def tune_class(ctx, name, kind, quick):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
-    ap.add_argument("--out", default=os.path.join(HERE, "ogs_tiles.txt"))
+    ap.add_argument("--out", default=os.path.join(HERE, "ogs_tiles.autotuned.txt"),
+                    help="tile file to write; a context applies it when $OGS_TILES names it")
     args = ap.parse_args()
-    os.environ["OGS_TILES"] = os.devnull  # start from the built-in defaults
+    os.environ.pop("OGS_TILES", None)  # start from the compiled-in table
     ctx = ogs.Context(0)
     lines = ["# libogs.so tile table written by autotune.py (kind param value)"]
     summary = {}

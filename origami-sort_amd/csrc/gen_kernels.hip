@@ -18,15 +18,18 @@ __device__ __forceinline__ uint64_t burst_of(const uint64_t *starts, uint64_t nb
     return lo - 1;
 }
 
+// W: item layout, 4 / 8 (keys), 16 ({u64 key, u64 index}) or kGenKv32
+// ({u32 key, u32 index}, the keys of the 4-byte layout)
 template <int W>
 __device__ __forceinline__ void gen_store(void *out, uint64_t i, int dist, uint64_t k, uint64_t n) {
-    if constexpr (W == 4) {
+    if constexpr (W == 4 || W == kGenKv32) {
         uint32_t v = gen::key32(dist, k, i, n);
         if (dist == gen::NORMAL) {
             if (i == n / 2) v = 0u;  // +0.0
             else if (i == 0) v = 0x80000000u;  // -0.0
         }
-        ((uint32_t *)out)[i] = v;
+        if constexpr (W == 4) ((uint32_t *)out)[i] = v;
+        else ((uint2 *)out)[i] = make_uint2(v, (uint32_t)i);
     } else if constexpr (W == 8) {
         ((uint64_t *)out)[i] = k;
     } else {
@@ -84,6 +87,8 @@ hipError_t launch_generate(int width, int dist, uint64_t seed, uint64_t n, void 
         hipLaunchKernelGGL(k_generate<4>, gen_grid(n), dim3(256), 0, s, dist, seed, n, out, cdf, starts, nb);
     else if (width == 8)
         hipLaunchKernelGGL(k_generate<8>, gen_grid(n), dim3(256), 0, s, dist, seed, n, out, cdf, starts, nb);
+    else if (width == kGenKv32)
+        hipLaunchKernelGGL(k_generate<kGenKv32>, gen_grid(n), dim3(256), 0, s, dist, seed, n, out, cdf, starts, nb);
     else
         hipLaunchKernelGGL(k_generate<16>, gen_grid(n), dim3(256), 0, s, dist, seed, n, out, cdf, starts, nb);
     return hipGetLastError();
@@ -102,6 +107,7 @@ hipError_t launch_generate_untag(int width, const void *pairs, uint64_t n, void 
     const ulonglong2 *p = (const ulonglong2 *)pairs;
     if (width == 4) hipLaunchKernelGGL(k_generate_untag<4>, gen_grid(n), dim3(256), 0, s, p, n, out);
     else if (width == 8) hipLaunchKernelGGL(k_generate_untag<8>, gen_grid(n), dim3(256), 0, s, p, n, out);
+    else if (width == kGenKv32) hipLaunchKernelGGL(k_generate_untag<kGenKv32>, gen_grid(n), dim3(256), 0, s, p, n, out);
     else hipLaunchKernelGGL(k_generate_untag<16>, gen_grid(n), dim3(256), 0, s, p, n, out);
     return hipGetLastError();
 }

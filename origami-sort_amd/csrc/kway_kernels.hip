@@ -277,7 +277,8 @@ __device__ __forceinline__ void store_items(const K *sp0, K *ob, int tid) {
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
                                                       int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
-                                                      const uint64_t *__restrict__ tab, SampleSpec sp) {
+                                                      const uint64_t *__restrict__ tab, SampleSpec sp,
+                                                      uint32_t *__restrict__ err) {
     static_assert(KW == 2 || KW == 4 || KW == 8 || KW == 16 || KW == 32, "fan-in 2, 4, 8, 16 or 32");
     static_assert((IPT & (IPT - 1)) == 0 && IPT >= 4, "IPT: a power of two >= 4");
     constexpr int C = BLOCK * IPT;
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
     __shared__ int s_P[KW + 1];     // prefix sums of the padded slice lengths
     __shared__ int s_R[KW + 1];     // prefix sums of the real slice lengths
     __shared__ uint64_t s_o0;       // output index of the tile's first item
-    __shared__ int s_cnt;           // items in the tile (-1: inconsistent table)
+    __shared__ int s_cnt;           // items in the tile (-1: inconsistent table, reported in *err)
     __shared__ int s_top[LOGKW + 1];  // per level: co-rank step bound over all pairs
     K *sk = skb + lds_front<K>();
     uint64_t *sv = svb + (HV ? lds_front<K>() : 0);
@@ -311,7 +312,8 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
             lo = tab[t * KW + j];
             const uint64_t hi = tl + 1 < nt_here ? tab[(t + 1) * KW + j] : lenj;
             s_src[j] = gbase + (uint64_t)j * R + lo;
-            // a split table that is not monotone gives a negative count: the tile is skipped
+            // a split table that is not monotone gives a negative count: the
+            // tile is not merged and the error word reports it
             nj = hi >= lo && hi <= lenj ? (int)(hi - lo) : -1;
             s_n[j] = nj;
         }
@@ -354,7 +356,10 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
     }
     __syncthreads();
     const int cnt = s_cnt;
-    if (cnt < 0) return;  // uniform
+    if (cnt < 0) {  // uniform: an inconsistent split table (never with a consistent one)
+        if (tid == 0) atomicOr(err, kErrKwaySplit);
+        return;
+    }
     const int Np = s_P[KW];
     const uint64_t o0 = s_o0;
     // this thread's chunk [c0, c0 + IPT) lies in slice j0: the last j with
@@ -545,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
             // store (the codec is uniform: one branch picks the loop)
             if (cout == 0) store_items<BLOCK, IPT, 0>(sp0, ob, tid);
             else if (cout == 1) store_items<BLOCK, IPT, 1>(sp0, ob, tid);
-            else store_items<BLOCK, IPT, 2>(sp0, ob, tid);
+            else store_items<BLOCK, IPT, sizeof(K) == 4 ? 2 : 3>(sp0, ob, tid);
         } else {
             const int nk = cnt > tid ? (cnt - tid + BLOCK - 1) / BLOCK : 0;
 #pragma unroll
@@ -558,7 +563,8 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
 
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
 static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, int slog2, int cout, uint64_t *tab,
-                              const void *s0, const void *sm, const SampleSpec *next, hipStream_t s, hipEvent_t mid) {
+                              const void *s0, const void *sm, const SampleSpec *next, hipStream_t s, hipEvent_t mid,
+                              uint32_t *err) {
     constexpr uint64_t C = (uint64_t)BLOCK * IPT;
     if (n == 0) return hipSuccess;
     const uint64_t Q = kw_q(C, IPT, KW, slog2);
@@ -573,7 +579,7 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     if (mid) (void)hipEventRecord(mid, s);
     const SampleSpec none = {nullptr, 0, 0, 0};
     hipLaunchKernelGGL((k_kway_merge<K, HV, KW, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n,
-                       rlog2, ntg, nt_last, cout, (const uint64_t *)tab, next ? *next : none);
+                       rlog2, ntg, nt_last, cout, (const uint64_t *)tab, next ? *next : none, err);
     return hipGetLastError();
 }
 

@@ -49,31 +49,83 @@ bool key_info(ogs_key_t kt, KeyInfo *ki) {
     case OGS_I64: *ki = {KC_64, 8, 1}; return true;
     case OGS_KV_U64U64: *ki = {KC_KV, 16, 0}; return true;
     case OGS_KV_I64I64: *ki = {KC_KV, 16, 1}; return true;
+    case OGS_KV_U32U32: *ki = {KC_64, 8, 3}; return true;
     default: return false;
     }
 }
 
-// Default tile table (overridden by the autotuner through ogs_tune_set).
-static Tuning default_tuning(KeyClass kc) {
+// Shape indices by shape: the compiled lists' positions of a block-sort shape
+// (block, ipt, wave levels), a merge shape and a K-way shape.  Absent shapes
+// give 0, the list's first entry.
+static int block_index(KeyClass kc, int block, int ipt, int wlev) {
+    int cnt = 0;
+    const BlockShapeFns *t = block_shapes(kc, &cnt);
+    for (int i = 0; i < cnt; i++)
+        if (t[i].block == block && t[i].ipt == ipt && t[i].wlev == wlev) return i;
+    return 0;
+}
+static int merge_index(KeyClass kc, int block, int ipt) {
+    int cnt = 0;
+    const MergeShapeFns *t = merge_shapes(kc, &cnt);
+    for (int i = 0; i < cnt; i++)
+        if (t[i].block == block && t[i].ipt == ipt) return i;
+    return 0;
+}
+static int kway_index(KeyClass kc, int block, int ipt) {
+    int cnt = 0;
+    const KwayShapeFns *t = kway_shapes(kc, &cnt);
+    for (int i = 0; i < cnt; i++)
+        if (t[i].block == block && t[i].ipt == ipt) return i;
+    return 0;
+}
+
+// The shipped tile table, measured on MI355X (the run-time counterpart of the
+// tuned constants of Origami/config.h:45-264).  A context starts from it;
+// ogs_tune_set / ogs_tune_load ($OGS_TILES, written by autotune.py) override
+// entries.
+Tuning default_tuning(KeyClass kc) {
     Tuning t;
-    (void)kc;
-    t.v[TP_PLAIN_BLOCK] = 0;
-    t.v[TP_PLAIN_MERGE] = 0;
-    t.v[TP_SEG_SMALL_BLOCK] = 0;
-    t.v[TP_SEG_SMALL_MAX] = 0;  // 0: the small class is the tile of its shape
-    t.v[TP_SEG_LARGE_BLOCK] = 0;
-    t.v[TP_SEG_MERGE] = 0;
-    t.v[TP_SEG_TINY_BLOCK] = 0;
-    t.v[TP_SEG_TINY_MAX] = 0;
     t.v[TP_PLAIN_CORANK] = 0;
-    t.v[TP_SEG_PLAIN_MIN] = 22;
+    t.v[TP_SEG_TINY_MAX] = 0;    // no tiny class
+    t.v[TP_SEG_SMALL_MAX] = 1024;
+    t.v[TP_SEG_PLAIN_MIN] = 0;   // every batch segment runs through the batched kernels
     t.v[TP_PARTITION_LANES] = 4;
-    t.v[TP_PLAIN_FANIN] = 2;
     t.v[TP_QUAD_LANES] = 1;
-    t.v[TP_KWAY_FANIN] = 0;
-    t.v[TP_KWAY_SHAPE] = 0;
-    t.v[TP_KWAY_SAMPLE] = 5;
-    t.v[TP_KWAY_SCORANK] = 0;
+    t.v[TP_KWAY_SCORANK] = 1;
+    if (kc == KC_32) {
+        t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 32, 4);
+        t.v[TP_PLAIN_MERGE] = merge_index(kc, 512, 32);
+        t.v[TP_SEG_TINY_BLOCK] = block_index(kc, 64, 16, 1);
+        t.v[TP_SEG_SMALL_BLOCK] = block_index(kc, 256, 8, 1);
+        t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 32, 4);
+        t.v[TP_SEG_MERGE] = merge_index(kc, 256, 16);
+        t.v[TP_PLAIN_FANIN] = 2;
+        t.v[TP_KWAY_FANIN] = 16;
+        t.v[TP_KWAY_SHAPE] = kway_index(kc, 512, 16);
+        t.v[TP_KWAY_SAMPLE] = 6;
+    } else if (kc == KC_64) {
+        t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 16, 2);
+        t.v[TP_PLAIN_MERGE] = merge_index(kc, 512, 16);
+        t.v[TP_SEG_TINY_BLOCK] = block_index(kc, 64, 8, 1);
+        t.v[TP_SEG_SMALL_BLOCK] = block_index(kc, 256, 8, 1);
+        t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 16, 2);
+        t.v[TP_SEG_MERGE] = merge_index(kc, 256, 16);
+        t.v[TP_PLAIN_FANIN] = 2;
+        t.v[TP_KWAY_FANIN] = 16;
+        t.v[TP_KWAY_SHAPE] = kway_index(kc, 512, 16);
+        t.v[TP_KWAY_SAMPLE] = 6;
+    } else {
+        t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 8, 3);
+        t.v[TP_PLAIN_MERGE] = merge_index(kc, 512, 4);
+        t.v[TP_SEG_TINY_BLOCK] = block_index(kc, 64, 8, 1);
+        t.v[TP_SEG_SMALL_BLOCK] = block_index(kc, 256, 4, 1);
+        t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 8, 3);
+        t.v[TP_SEG_MERGE] = merge_index(kc, 512, 4);
+        t.v[TP_PLAIN_FANIN] = 4;
+        t.v[TP_KWAY_FANIN] = 8;
+        t.v[TP_KWAY_SHAPE] = kway_index(kc, 512, 4);
+        t.v[TP_KWAY_SAMPLE] = 5;
+    }
     return t;
 }
 
@@ -272,7 +324,8 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
             }
             if (wy == 4)
                 e = ms.run_quad(src, last && vout ? kout : dst, n, R, last ? ki.codec : 0, part,
-                                tune[ki.kc].v[TP_QUAD_LANES], s, prof.on ? ev[1] : nullptr, last ? vout : nullptr);
+                                tune[ki.kc].v[TP_QUAD_LANES], s, prof.on ? ev[1] : nullptr, last ? vout : nullptr,
+                                err);
             else
                 e = ms.run(src, last && vout ? kout : dst, n, R, last ? ki.codec : 0, table ? part : nullptr,
                            tune[ki.kc].v[TP_PARTITION_LANES], s, prof.on ? ev[1] : nullptr, last ? vout : nullptr);
@@ -427,7 +480,7 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
         SampleSpec next = {nullptr, 0, 0, 0};
         if (!last) next = {S0, rl + lv[p], 1 << lv[p + 1], sl[p + 1]};
         e = ks.run[lv[p] - 1](src, dst, n, rl, sl[p], last ? ki.codec : 0, tab, S0, sa, last ? nullptr : &next, s,
-                              prof.on ? ev[1] : nullptr);
+                              prof.on ? ev[1] : nullptr, err);
         if (e != hipSuccess) return hip_fail(e, "K-way pass launch");
         if (prof.on) {
             (void)hipEventRecord(ev[2], s);
@@ -686,11 +739,27 @@ int ogs_ctx_create(ogs_ctx **ctx, int device) {
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
     if (device < 0 || device >= ndev) return fail(OGS_ERR_ARG, "device %d out of range (%d devices)", device, ndev);
+    e = hipSetDevice(device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    uint32_t *err = nullptr;
+    e = hipMalloc(&err, 256);
+    if (e == hipSuccess) e = hipMemset(err, 0, 256);
+    if (e != hipSuccess) {
+        if (err) (void)hipFree(err);
+        return fail(OGS_ERR_NOMEM, "device error word: %s", hipGetErrorString(e));
+    }
     Ctx *c = new Ctx();
     c->device = device;
+    c->err = err;
     for (int k = 0; k < 3; k++) c->tune[k] = default_tuning((KeyClass)k);
     *ctx = reinterpret_cast<ogs_ctx *>(c);
-    load_default_tiles(*ctx);
+    const int rc = load_env_tiles(*ctx);  // a bad $OGS_TILES fails the call
+    if (rc) {
+        const std::string msg = ogs_last_error();
+        ogs_ctx_destroy(*ctx);
+        *ctx = nullptr;
+        return fail(rc, "%s", msg.c_str());
+    }
     return OGS_OK;
 }
 
@@ -702,7 +771,23 @@ int ogs_ctx_destroy(ogs_ctx *ctx) {
     for (auto &kv : c->ws) delete kv.second;
     if (c->host_buf[0]) (void)hipFree(c->host_buf[0]);
     if (c->host_buf[1]) (void)hipFree(c->host_buf[1]);
+    if (c->err) (void)hipFree(c->err);
     delete c;
+    return OGS_OK;
+}
+
+int ogs_device_errors(ogs_ctx *ctx, uint32_t *flags) {
+    if (!ctx || !flags) return fail(OGS_ERR_ARG, "NULL argument");
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    *flags = 0;
+    hipError_t e = hipSetDevice(c->device);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(flags, c->err, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && *flags) {
+        const uint32_t zero = 0;
+        e = hipMemcpy(c->err, &zero, sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) return hip_fail(e, "device error word");
     return OGS_OK;
 }
 
@@ -938,8 +1023,9 @@ int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n
         e = upload(st.data(), st.size() * sizeof(uint64_t), (void **)&d_starts);
         if (e != hipSuccess) return done(hip_fail(e, "burst table upload"));
     }
+    const int layout = kt == OGS_KV_U32U32 ? kGenKv32 : ki.width;
     if (d != gen::PARETO_SHUFFLED) {
-        e = launch_generate(ki.width, d, seed, n, d_out, d_cdf, d_starts, nb, s);
+        e = launch_generate(layout, d, seed, n, d_out, d_cdf, d_starts, nb, s);
         return done(e == hipSuccess ? OGS_OK : hip_fail(e, "generator launch"));
     }
     // shuffled bursts: (tag, key) pairs sorted by tag with this library's sort
@@ -957,7 +1043,7 @@ int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n
     void *sorted = nullptr;
     int rc = c->sort(OGS_KV_U64U64, nullptr, p0, p1, n, &sorted, s);
     if (rc) return done(rc);
-    e = launch_generate_untag(ki.width, sorted, n, d_out, s);
+    e = launch_generate_untag(layout, sorted, n, d_out, s);
     return done(e == hipSuccess ? OGS_OK : hip_fail(e, "generator launch"));
 }
 
@@ -1007,6 +1093,12 @@ int ogs_sort_host(ogs_ctx *ctx, ogs_key_t kt, void *h_data, uint64_t n, uint64_t
     if (rc) return rc;
     e = hipMemcpy(h_data, res, bytes, hipMemcpyDeviceToHost);
     if (e != hipSuccess) return hip_fail(e, "D2H copy");
+    uint32_t flags = 0;
+    rc = ogs_device_errors(ctx, &flags);
+    if (rc) return rc;
+    if (flags)
+        return fail(OGS_ERR_STATE, "a merge kernel reported an inconsistent tile split table (device error word 0x%x)",
+                    flags);
     return OGS_OK;
 }
 

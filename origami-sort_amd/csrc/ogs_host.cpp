@@ -1,7 +1,6 @@
 // ogs_host.cpp -- host-side parts of the C-ABI that launch nothing: the seeded
 // input generator (the datagen::Writer analogue, Origami/writer.h:47-242) and
 // the tile-table accessors used by the autotuner.
-#include <dlfcn.h>
 
 #include <algorithm>
 #include <cmath>
@@ -132,6 +131,19 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
     const int base = dist & ~OGS_GEN_COUNTER;
     if (base < 0 || base >= gen::COUNT) return fail(OGS_ERR_ARG, "unknown distribution %d", dist);
     if (n && !h_out) return fail(OGS_ERR_ARG, "h_out is NULL");
+    if (kt == OGS_KV_U32U32) {
+        // {u32 key, u32 value}: the four-byte keys of the same distribution,
+        // value = original index
+        std::vector<uint32_t> k(n);
+        const int rc = ogs_generate_host(OGS_U32, dist, seed, n, k.data());
+        if (rc) return rc;
+        uint32_t *o = (uint32_t *)h_out;
+        for (uint64_t i = 0; i < n; i++) {
+            o[2 * i] = k[i];
+            o[2 * i + 1] = (uint32_t)i;
+        }
+        return OGS_OK;
+    }
     if ((dist & OGS_GEN_COUNTER) || base >= gen::SAME) return generate_counter_host(ki, base, seed, n, h_out);
     const int w = ki.width;
     if (w == 4 && dist == 0) {
@@ -201,52 +213,49 @@ int ogs_generate_host(ogs_key_t kt, int dist, uint64_t seed, uint64_t n, void *h
     return OGS_OK;
 }
 
-// Tile-table access (no reference counterpart beyond the compile-time
-// constants of Origami/config.h).  `param` is one of plain_block, plain_merge,
-// seg_small_block, seg_small_max, seg_large_block, seg_merge.
-int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
-    if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
-    KeyInfo ki;
-    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+}  // extern "C"
+
+namespace ogs {
+// Sets one tile-table entry of t (the table of ki's key class) after checking
+// the value against the parameter's range.
+static int tune_set_into(Tuning *t, const KeyInfo &ki, const char *param, int value) {
     const int p = param_index(param);
     if (p < 0) return fail(OGS_ERR_ARG, "unknown tuning parameter '%s'", param ? param : "(null)");
     int cnt = 0;
     const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX || p == TP_SEG_PLAIN_MIN;
+    bool ok = true;
     if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
     else if (p == TP_PLAIN_CORANK || p == TP_KWAY_SCORANK) cnt = 2;
-    else if (p == TP_PLAIN_FANIN) {
-        if (value != 2 && value != 4) return fail(OGS_ERR_ARG, "plain_fanin must be 2 or 4 (got %d)", value);
-        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
-        return OGS_OK;
-    }
-    else if (p == TP_QUAD_LANES) {
-        if (value != 1 && value != 2 && value != 4 && value != 8)
-            return fail(OGS_ERR_ARG, "quad_lanes must be 1, 2, 4 or 8 (got %d)", value);
-        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
-        return OGS_OK;
-    }
-    else if (p == TP_KWAY_FANIN) {
-        if (value != 0 && value != 2 && value != 4 && value != 8 && value != 16 && value != 32)
-            return fail(OGS_ERR_ARG, "kway_fanin must be 0, 2, 4, 8, 16 or 32 (got %d)", value);
-        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
-        return OGS_OK;
-    }
-    else if (p == TP_KWAY_SAMPLE) {
-        if (value < 3 || value > 10) return fail(OGS_ERR_ARG, "kway_sample must be in [3, 10] (got %d)", value);
-        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
-        return OGS_OK;
-    }
+    else if (p == TP_PLAIN_FANIN) ok = value == 2 || value == 4;
+    else if (p == TP_QUAD_LANES) ok = value == 1 || value == 2 || value == 4 || value == 8;
+    else if (p == TP_KWAY_FANIN) ok = value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32;
+    else if (p == TP_KWAY_SAMPLE) ok = value >= 3 && value <= 10;
     else if (p == TP_KWAY_SHAPE) kway_shapes(ki.kc, &cnt);
-    else if (p == TP_PARTITION_LANES) {
-        if (value != 1 && value != 4 && value != 8 && value != 16)
-            return fail(OGS_ERR_ARG, "partition_lanes must be 1, 4, 8 or 16 (got %d)", value);
-        reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
-        return OGS_OK;
-    }
+    else if (p == TP_PARTITION_LANES) ok = value == 1 || value == 4 || value == 8 || value == 16;
     else if (!is_count) block_shapes(ki.kc, &cnt);
-    if (is_count ? value < 0 : (value < 0 || value >= cnt))
-        return fail(OGS_ERR_ARG, "value %d out of range for '%s'", value, param);
-    reinterpret_cast<Ctx *>(ctx)->tune[ki.kc].v[p] = value;
+    if (cnt) ok = value >= 0 && value < cnt;
+    else if (is_count) ok = value >= 0;
+    if (!ok) return fail(OGS_ERR_ARG, "value %d out of range for '%s'", value, param);
+    t[ki.kc].v[p] = value;
+    return OGS_OK;
+}
+}  // namespace ogs
+
+extern "C" {
+
+// Tile-table access (no reference counterpart beyond the compile-time
+// constants of Origami/config.h).
+int ogs_tune_set(ogs_ctx *ctx, ogs_key_t kt, const char *param, int value) {
+    if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
+    KeyInfo ki;
+    if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
+    return tune_set_into(reinterpret_cast<Ctx *>(ctx)->tune, ki, param, value);
+}
+
+int ogs_tune_reset(ogs_ctx *ctx) {
+    if (!ctx) return fail(OGS_ERR_ARG, "ctx is NULL");
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    for (int k = 0; k < 3; k++) c->tune[k] = default_tuning((KeyClass)k);
     return OGS_OK;
 }
 
@@ -261,13 +270,17 @@ int ogs_tune_get(ogs_ctx *ctx, ogs_key_t kt, const char *param, int *value) {
 }
 
 // Tile-table files: one "kind param value" triple per line, '#' comments.
-// kind is u32/i32/f32/u64/i64/kv_u64u64/kv_i64i64 (the classes share tables:
-// u32/i32/f32, u64/i64, kv_*).
+// kind is u32/i32/f32/u64/i64/kv_u64u64/kv_i64i64/kv_u32u32 (the classes share
+// tables: u32/i32/f32, u64/i64/kv_u32u32, kv_u64u64/kv_i64i64).  The file is
+// applied all or nothing: any bad line leaves the table as it was.
 int ogs_tune_load(ogs_ctx *ctx, const char *path) {
     if (!ctx || !path) return fail(OGS_ERR_ARG, "NULL argument");
     FILE *f = fopen(path, "r");
     if (!f) return fail(OGS_ERR_ARG, "cannot open tile table '%s'", path);
-    static const char *kinds[] = {"u32", "i32", "u64", "i64", "f32", "kv_u64u64", "kv_i64i64"};
+    static const char *kinds[] = {"u32", "i32", "u64", "i64", "f32", "kv_u64u64", "kv_i64i64", "kv_u32u32"};
+    constexpr int NK = (int)(sizeof(kinds) / sizeof(kinds[0]));
+    Ctx *c = reinterpret_cast<Ctx *>(ctx);
+    Tuning t[3] = {c->tune[0], c->tune[1], c->tune[2]};
     char line[256];
     int lineno = 0, rc = OGS_OK;
     while (fgets(line, sizeof line, f)) {
@@ -280,13 +293,21 @@ int ogs_tune_load(ogs_ctx *ctx, const char *path) {
         if (got <= 0) continue;
         if (got != 3) { rc = fail(OGS_ERR_ARG, "%s:%d: expected 'kind param value'", path, lineno); break; }
         int kt = -1;
-        for (int i = 0; i < 7; i++)
+        for (int i = 0; i < NK; i++)
             if (!strcmp(k, kinds[i])) kt = i;
         if (kt < 0) { rc = fail(OGS_ERR_ARG, "%s:%d: unknown kind '%s'", path, lineno, k); break; }
-        rc = ogs_tune_set(ctx, (ogs_key_t)kt, p, v);
-        if (rc) break;
+        KeyInfo ki;
+        key_info((ogs_key_t)kt, &ki);
+        rc = tune_set_into(t, ki, p, v);
+        if (rc) {
+            std::string msg = ogs_last_error();
+            rc = fail(rc, "%s:%d: %s", path, lineno, msg.c_str());
+            break;
+        }
     }
     fclose(f);
+    if (rc == OGS_OK)
+        for (int i = 0; i < 3; i++) c->tune[i] = t[i];
     return rc;
 }
 
@@ -324,24 +345,13 @@ int ogs_shape_info(ogs_key_t kt, int merge, int idx, int *block, int *ipt, int *
 }  // extern "C"
 
 namespace ogs {
-// Applies $OGS_TILES, else ogs_tiles.txt beside libogs.so, when present.
-void load_default_tiles(ogs_ctx *ctx) {
+// Applies $OGS_TILES when it is set: a context starts from the compiled-in
+// table, and a tile file is an explicit override (autotune.py writes one).
+int load_env_tiles(ogs_ctx *ctx) {
     const char *env = getenv("OGS_TILES");
-    std::string path;
-    if (env && *env) {
-        path = env;
-    } else {
-        Dl_info info;
-        if (dladdr((void *)&ogs_tune_load, &info) && info.dli_fname) {
-            path = info.dli_fname;
-            const size_t slash = path.rfind('/');
-            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/ogs_tiles.txt";
-        }
-    }
-    if (path.empty()) return;
-    FILE *f = fopen(path.c_str(), "r");
-    if (!f) return;
-    fclose(f);
-    (void)ogs_tune_load(ctx, path.c_str());
+    if (!env || !*env) return OGS_OK;
+    const int rc = ogs_tune_load(ctx, env);
+    if (rc) std::fprintf(stderr, "libogs: OGS_TILES not applied: %s\n", ogs_last_error());
+    return rc;
 }
 }  // namespace ogs

@@ -19,7 +19,7 @@ int hip_fail(hipError_t e, const char *what);
 struct KeyInfo {
     KeyClass kc;
     int width;  // bytes per item
-    int codec;  // 0 identity, 1 sign flip, 2 float totalOrder
+    int codec;  // 0 identity, 1 sign flip, 2 float totalOrder, 3 swap the 32-bit halves (8-byte records)
 };
 bool key_info(ogs_key_t kt, KeyInfo *ki);
 
@@ -129,6 +129,7 @@ struct BatchBuild {
 
 struct Ctx {
     int device = 0;
+    uint32_t *err = nullptr;  // device error word (kErr* bits), read and cleared by ogs_device_errors
     Profiler prof;
     Tuning tune[3];
     std::mutex mu;  // guards ws
@@ -161,7 +162,9 @@ struct Ctx {
     int pass_count(ogs_key_t kt, uint64_t n);
 };
 
-void load_default_tiles(ogs_ctx *ctx);
+Tuning default_tuning(KeyClass kc);
+// Applies the tile-table file named by $OGS_TILES, when set (all or nothing).
+int load_env_tiles(ogs_ctx *ctx);
 int csr_segments(const KeyInfo &ki, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg,
                  std::vector<SegSpec> &segs);
 // Counter-based generator tables and host mirror (ogs_host.cpp).

@@ -18,9 +18,12 @@ namespace ogs {
 
 // ----------------------------------------------------------------------------
 // Key codecs.  0: identity, 1: flip the sign bit (two's-complement order),
-// 2: float32 IEEE totalOrder.  The reference compares u32 unsigned
-// (avx2_utils.h:129-133) and i64 signed (avx2_utils.h:140-145).
+// 2: float32 IEEE totalOrder, 3 (8-byte items only): swap the two 32-bit
+// halves, so an 8-byte {u32 key, u32 value} record (little-endian: the key in
+// the low half) sorts as the u64 key << 32 | value.  The reference compares
+// u32 unsigned (avx2_utils.h:129-133) and i64 signed (avx2_utils.h:140-145).
 // ----------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t swap_halves(uint64_t x) { return (x << 32) | (x >> 32); }
 __device__ __forceinline__ uint32_t enc_key(uint32_t x, int c) {
     if (c == 1) return x ^ 0x80000000u;
     if (c == 2) return x ^ ((uint32_t)((int32_t)x >> 31) | 0x80000000u);
@@ -31,32 +34,41 @@ __device__ __forceinline__ uint32_t dec_key(uint32_t x, int c) {
     if (c == 2) return x ^ (~(uint32_t)((int32_t)x >> 31) | 0x80000000u);
     return x;
 }
-__device__ __forceinline__ uint64_t enc_key(uint64_t x, int c) { return c == 1 ? x ^ (1ull << 63) : x; }
-__device__ __forceinline__ uint64_t dec_key(uint64_t x, int c) { return c == 1 ? x ^ (1ull << 63) : x; }
+__device__ __forceinline__ uint64_t enc_key(uint64_t x, int c) {
+    return c == 1 ? x ^ (1ull << 63) : c == 3 ? swap_halves(x) : x;
+}
+__device__ __forceinline__ uint64_t dec_key(uint64_t x, int c) { return enc_key(x, c); }
 
 // The same codecs without branches, for loops over many keys with one codec:
 // enc(x) = x ^ ((sx & a) | b), dec(x) = x ^ ((~sx & a) | b), with sx all ones
 // when the top bit of x is set; (a, b) = (0, 0) identity, (0, top) sign flip,
-// (~0, top) float32 totalOrder.
+// (~0, top) float32 totalOrder; `rot` swaps the halves of an 8-byte key
+// (codec 3, an involution).
 template <typename K> struct CodecMask {
     K a, b;
+    bool rot;
 };
 template <typename K> __device__ __forceinline__ CodecMask<K> codec_mask(int c) {
     constexpr K TOP = (K)1 << (sizeof(K) * 8 - 1);
     CodecMask<K> m;
     m.a = (sizeof(K) == 4 && c == 2) ? (K)~(K)0 : (K)0;
-    m.b = c != 0 ? TOP : (K)0;
+    m.b = (c == 1 || c == 2) ? TOP : (K)0;
+    m.rot = sizeof(K) == 8 && c == 3;
     return m;
 }
 template <typename K> __device__ __forceinline__ K sign_all(K x) {
     if constexpr (sizeof(K) == 4) return (K)((int32_t)x >> 31);
     else return (K)((int64_t)x >> 63);
 }
+template <typename K> __device__ __forceinline__ K rot_if(K x, bool rot) {
+    if constexpr (sizeof(K) == 8) return rot ? (K)swap_halves((uint64_t)x) : x;
+    else return x;
+}
 template <typename K> __device__ __forceinline__ K enc_bf(K x, CodecMask<K> m) {
-    return x ^ ((sign_all(x) & m.a) | m.b);
+    return rot_if(x, m.rot) ^ ((sign_all(x) & m.a) | m.b);
 }
 template <typename K> __device__ __forceinline__ K dec_bf(K x, CodecMask<K> m) {
-    return x ^ ((~sign_all(x) & m.a) | m.b);
+    return rot_if(x ^ ((~sign_all(x) & m.a) | m.b), m.rot);
 }
 
 template <typename K> struct KeyMax;
@@ -102,31 +114,9 @@ struct GMem {
 // stores then hits distinct banks from its neighbours (plain layout: lanes 0
 // and 2 collide), and lane-consecutive accesses stay a permutation within a
 // row.  Items of width T; the swizzle never moves an item out of its chunk.
-#ifndef OGS_LDS_SWIZZLE
-#define OGS_LDS_SWIZZLE 1
-#endif
-// 1: LDS merges use per-thread co-ranks plus a bitonic slice merge
-// (lds_merge_bitonic); 0: serial merge chains (lds_merge_chains).
-#ifndef OGS_MERGE_BITONIC
-#define OGS_MERGE_BITONIC 0
-#endif
-// 1: key-only LDS merges use the bitonic layout (B stored descending) and skip
-// the exhaustion tests; 0: every merge uses the ascending layout.
-#ifndef OGS_REV_MERGE
-#define OGS_REV_MERGE 1
-#endif
-// 1: the full-slice merge on the bitonic layout keeps (loser, newcomer) heads
-// instead of (A head, B head).
-#ifndef OGS_MERGE_LOSER
-#define OGS_MERGE_LOSER 1
-#endif
 template <typename T> __device__ __forceinline__ int lsw(int i) {
-#if OGS_LDS_SWIZZLE
     if constexpr (sizeof(T) == 4) return i ^ (((i >> 5) & 7) << 2);
     else return i ^ (((i >> 4) & 7) << 1);
-#else
-    return i;
-#endif
 }
 // items per 16-byte chunk
 template <typename T> constexpr int chunk_items() { return 16 / (int)sizeof(T); }
@@ -134,13 +124,7 @@ template <typename T> constexpr int chunk_items() { return 16 / (int)sizeof(T); 
 // The swizzle on byte offsets (for 4- and 8-byte items alike: the 16-byte
 // chunk index within a 128-byte row XOR the row number): lets merge loops keep
 // their pointers in bytes and address LDS with two instructions.
-__device__ __forceinline__ int lsw_bytes(int b) {
-#if OGS_LDS_SWIZZLE
-    return b ^ ((b >> 3) & 0x70);
-#else
-    return b;
-#endif
-}
+__device__ __forceinline__ int lsw_bytes(int b) { return b ^ ((b >> 3) & 0x70); }
 template <typename K> __device__ __forceinline__ K lds_at_bytes(const K *sk, int b) {
     return *(const K *)((const char *)sk + lsw_bytes(b));
 }
@@ -326,38 +310,6 @@ template <typename K> constexpr int lds_front() { return 128 / (int)sizeof(K); }
 template <typename K, int IPT> constexpr int lds_back() { return IPT + 32; }
 template <typename K, int IPT> constexpr int lds_extra() { return lds_front<K>() + lds_back<K, IPT>(); }
 
-// Merges this thread's slice of merge(A, B) -- `na` items of A starting at
-// LDS index ia, and IPT - na items of B ending at LDS index jb - na (inclusive)
-// -- into IPT sorted registers.  The slice is laid out as A ascending followed
-// by B descending (w[i] = i < na ? A[ia + i] : B[jb - i]), which is bitonic, and
-// sorted by a log2(IPT)-stage bitonic merge.  All loads are independent of one
-// another (no per-output dependency chain), and with an unswizzled layout they
-// are base + constant offsets.  Ties between equal keys may leave in either
-// order; the set of items is exactly the slice.
-template <int IPT, bool HV, typename K>
-__device__ __forceinline__ void lds_merge_bitonic(const K *sk, const uint64_t *sv, int ia, int na, int jb,
-                                                  K (&r)[IPT], uint64_t (&v)[IPT]) {
-    static_assert((IPT & (IPT - 1)) == 0, "bitonic slice merge needs a power-of-two IPT");
-#pragma unroll
-    for (int i = 0; i < IPT; i++) {
-        if constexpr (HV) {
-            const int idx = i < na ? ia + i : jb - i;
-            r[i] = sk[lsw<K>(idx)];
-            v[i] = sv[lsw<uint64_t>(idx)];
-        } else {
-            const K ka = sk[lsw<K>(ia + i)];
-            const K kb = sk[lsw<K>(jb - i)];
-            r[i] = i < na ? ka : kb;
-            v[i] = 0;
-        }
-    }
-#pragma unroll
-    for (int s = IPT / 2; s > 0; s >>= 1)
-#pragma unroll
-        for (int i = 0; i < IPT; i++)
-            if (!(i & s)) cex<HV>(r[i], r[i + s], v[i], v[i + s]);
-}
-
 // Serial branch-free merge of outputs into registers, starting at A index ai
 // and B index bi: all IPT outputs when FULL, else r[i0 .. i0+cnt) (i0 + cnt <=
 // IPT).  Reads one LDS item per output, with no clamp: reading one item past
@@ -484,7 +436,6 @@ __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i
         constexpr int W = (int)sizeof(K);
         const int dlt = (ib - ia) * W;
         int pa = ia * W;
-#if OGS_MERGE_LOSER
         // Key-only, so equal keys are identical bits and ties may go either
         // way: the registers hold the head that lost the last compare (L) and
         // the item read last (N, from run A when nA), the output is min(N, L),
@@ -507,21 +458,6 @@ __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i
             nA = fromA;
         }
         return;
-#else
-#pragma unroll
-        for (int i = 0; i < IPT; i++) {
-            const bool takeA = !(kb < ka);
-            r[i] = takeA ? ka : kb;
-            const int pa1 = pa + W;
-            const int nb = pa + dlt - W * (i + 1);
-            const int nx = takeA ? pa1 : nb;
-            pa = takeA ? pa1 : pa;
-            const K nk = lds_at_bytes(sk, nx);
-            ka = takeA ? nk : ka;
-            kb = takeA ? kb : nk;
-        }
-        return;
-#endif
     }
 #pragma unroll
     for (int i = 0; i < IPT; i++) {

@@ -23,7 +23,7 @@ namespace ogs {
 // ----------------------------------------------------------------------------
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t base, int cnt, int cin, int cout,
-                                          K *sk, uint64_t *sv, int *smisc, int *sa, const void *vin = nullptr,
+                                          K *sk, uint64_t *sv, int *smisc, const void *vin = nullptr,
                                           void *vout = nullptr, const SampleSpec *sp = nullptr) {
     constexpr int TILE = BLOCK * IPT;
     constexpr K KMAX = KeyMax<K>::v;
@@ -112,27 +112,17 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
         const int pb = o & ~(2 * R - 1);
         const int diag = o - pb;
         __syncthreads();
-#if OGS_REV_MERGE && !OGS_MERGE_BITONIC
         if constexpr (!HV) {
             // odd runs of length R are the B side of this round: descending
             if (o & R) lds_put_desc<IPT>(sk, 2 * (o & ~(R - 1)) + R - 1 - o, r);
             else lds_put<IPT, HV>(sk, sv, o, r, v);
             __syncthreads();
             lds_merge_slice_rev<IPT>(sk, pb, R, pb + 2 * R - 1, R, diag, IPT, R, r);
-            continue;
+        } else {
+            lds_put<IPT, HV>(sk, sv, o, r, v);
+            __syncthreads();
+            lds_merge_slice<IPT, HV>(sk, sv, pb, R, pb + R, R, diag, IPT, R, r, v);
         }
-#endif
-        lds_put<IPT, HV>(sk, sv, o, r, v);
-        __syncthreads();
-#if OGS_MERGE_BITONIC
-        const int a = lds_corank_lift<K>(sk, pb, R, pb + R, R, diag, R);
-        sa[tid] = a;
-        __syncthreads();
-        const int ae = diag + IPT == 2 * R ? R : sa[tid + 1];
-        lds_merge_bitonic<IPT, HV>(sk, sv, pb + a, ae - a, pb + R + diag - a + IPT - 1, r, v);
-#else
-        lds_merge_slice<IPT, HV>(sk, sv, pb, R, pb + R, R, diag, IPT, R, r, v);
-#endif
     }
     __syncthreads();
     lds_put<IPT, HV>(sk, sv, tid * IPT, r, v);
@@ -150,11 +140,10 @@ __global__ __launch_bounds__(BLOCK) void k_block_sort(const void *in, void *out,
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ int smisc[2];
-    __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
     const uint64_t base = (uint64_t)blockIdx.x * TILE;
     const int cnt = (int)(n - base < (uint64_t)TILE ? n - base : (uint64_t)TILE);
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc, sa, vin, vout, &sp);
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(in, out, base, cnt, cin, cout, sk, sv, smisc, vin, vout, &sp);
 }
 
 // Batched tile sort: descriptor i sorts one tile -- a whole small segment, or
@@ -175,12 +164,10 @@ __global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const TileDesc *__restr
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ int smisc[2];
-    __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
     const TileDesc t = tiles[blockIdx.x];
     const int cnt = (int)(t.cnt & 0x3FFFFFFFu);
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, ((t.cnt >> 30) & 1u) ? cout : 0, sk, sv, smisc,
-                                       sa);
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, ((t.cnt >> 30) & 1u) ? cout : 0, sk, sv, smisc);
 }
 
 // ----------------------------------------------------------------------------
@@ -258,7 +245,7 @@ __global__ void k_merge_partition(const void *__restrict__ in, uint64_t n, uint6
 template <typename K, bool HV, int BLOCK, int IPT>
 __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__restrict__ out, uint64_t seg,
                                            uint64_t n, uint64_t R, uint64_t o0, int cout, K *sk, uint64_t *sv,
-                                           uint64_t *sco, int *sa, const uint64_t *part, void *vout = nullptr) {
+                                           uint64_t *sco, const uint64_t *part, void *vout = nullptr) {
     static_assert(BLOCK >= 128, "the co-rank searches need two waves");
     constexpr int TILE = BLOCK * IPT;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -303,12 +290,8 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
         for (int i = 0; i < IPT; i++) {
             const int e = i * BLOCK + tid;
             if (e < cnt) {
-#if OGS_REV_MERGE && !OGS_MERGE_BITONIC
                 // key-only: the B slice descending after A (bitonic layout)
                 if constexpr (!HV) sk[lsw<K>(e < la ? e : cnt - 1 + la - e)] = kk[i];
-#else
-                if constexpr (!HV) sk[lsw<K>(e)] = kk[i];
-#endif
                 if constexpr (HV) {
                     sk[lsw<K>(e)] = kk[i];
                     sv[lsw<uint64_t>(e)] = vv[i];
@@ -320,34 +303,17 @@ __device__ __forceinline__ void merge_tile(const void *__restrict__ in, void *__
     const int d = tid * IPT;
     K r[IPT];
     uint64_t v[IPT];
-#if OGS_REV_MERGE && !OGS_MERGE_BITONIC
     if constexpr (!HV) {
         if (d < cnt)
             lds_merge_slice_rev<IPT>(sk, 0, la, cnt - 1, lb, d, cnt - d < IPT ? cnt - d : IPT,
                                      pow2_floor(la < lb ? la : lb), r);
 #pragma unroll
         for (int i = 0; i < IPT; i++) v[i] = 0;
-    } else
-#endif
-#if OGS_MERGE_BITONIC
-    // One co-rank per thread, exchanged through LDS so each thread knows where
-    // its slice ends; the one thread whose slice is partial merges serially.
-    int top = la < lb ? la : lb;
-    top = top ? 1 << (31 - __clz(top)) : 0;
-    const int a = d < cnt ? lds_corank_lift<K>(sk, 0, la, la, lb, d, top) : 0;
-    sa[tid] = a;
-    __syncthreads();
-    if (d + IPT <= cnt) {
-        const int ae = d + IPT == cnt ? la : sa[tid + 1];
-        lds_merge_bitonic<IPT, HV>(sk, sv, a, ae - a, la + d - a + IPT - 1, r, v);
-    } else if (d < cnt) {
-        lds_merge_serial<IPT, HV, false>(sk, sv, 0, la, la, lb, a, d - a, 0, cnt - d, r, v);
+    } else {
+        if (d < cnt)
+            lds_merge_slice<IPT, HV>(sk, sv, 0, la, la, lb, d, cnt - d < IPT ? cnt - d : IPT,
+                                     pow2_floor(la < lb ? la : lb), r, v);
     }
-#else
-    if (d < cnt)
-        lds_merge_slice<IPT, HV>(sk, sv, 0, la, la, lb, d, cnt - d < IPT ? cnt - d : IPT, pow2_floor(la < lb ? la : lb),
-                                 r, v);
-#endif
     __syncthreads();
     if (d < cnt) lds_put<IPT, HV>(sk, sv, d, r, v);
     __syncthreads();
@@ -362,9 +328,8 @@ __global__ __launch_bounds__(BLOCK) void k_merge_pass(const void *__restrict__ i
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ uint64_t sco[2];
-    __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
-    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco, sa,
+    merge_tile<K, HV, BLOCK, IPT>(in, out, 0, n, R, (uint64_t)blockIdx.x * TILE, cout, sk, sv, sco,
                                   part ? part + blockIdx.x : nullptr, vout);
 }
 
@@ -407,12 +372,11 @@ __global__ __launch_bounds__(BLOCK) void k_seg_merge_pass(const MergeDesc *__res
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ uint64_t sco[2];
-    __shared__ int sa[BLOCK + 1];
     K *sk = skb + lds_front<K>();
     const MergeDesc d = tiles[blockIdx.x];
     const int co = (d.o0 & kLastBit) ? cout : 0;
     const uint64_t o0 = d.o0 & ~kLastBit;
-    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco, sa,
+    merge_tile<K, HV, BLOCK, IPT>(d.src, d.dst, 0, d.len, R, o0, co, sk, sv, sco,
                                   part ? part + blockIdx.x : nullptr);
 }
 
@@ -757,7 +721,8 @@ __device__ __forceinline__ void quad_level1(const K *sk, const uint64_t *sv, int
 template <typename K, bool HV, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restrict__ in, void *__restrict__ out,
                                                            uint64_t n, uint64_t R, int cout,
-                                                           const uint64_t *__restrict__ tab, void *vout) {
+                                                           const uint64_t *__restrict__ tab, void *vout,
+                                                           uint32_t *__restrict__ err) {
     constexpr int C = BLOCK * IPT;  // outputs per round
     constexpr int CAP = 2 * C;      // LDS items: two intervals of at most C
     constexpr uint64_t h = C / 2;
@@ -782,9 +747,13 @@ __global__ __launch_bounds__(BLOCK) void k_quad_merge_pass(const void *__restric
         d1 = t0[11];
     }
     // a split table that is not monotone or overfills the tile is never read
-    // past: such a tile is skipped (its output stays unwritten)
-    if (a0 > a1 || a1 > q.la || b0 > b1 || b1 > q.lb || c0 > c1 || c1 > q.lc || d0 > d1 || d1 > q.ld) return;
-    if ((a1 - a0) + (b1 - b0) + (c1 - c0) + (d1 - d0) > (uint64_t)CAP) return;
+    // past: such a tile is not merged and the error word reports it (never
+    // with a consistent table)
+    if (a0 > a1 || a1 > q.la || b0 > b1 || b1 > q.lb || c0 > c1 || c1 > q.lc || d0 > d1 || d1 > q.ld ||
+        (a1 - a0) + (b1 - b0) + (c1 - c0) + (d1 - d0) > (uint64_t)CAP) {
+        if (tid == 0) atomicOr(err, kErrQuadSplit);
+        return;
+    }
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0), nc = (int)(c1 - c0), nd = (int)(d1 - d0);
     const int cnt = na + nb + nc + nd;
     if (cnt == 0) return;
@@ -850,12 +819,13 @@ __host__ __device__ inline uint64_t quad_pass_slots(uint64_t n, uint64_t R, uint
 // loads).  `mid` is recorded between the two.
 template <typename K, bool HV, int BLOCK, int IPT>
 static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
-                              int quad_lanes, hipStream_t s, hipEvent_t mid, void *vout) {
+                              int quad_lanes, hipStream_t s, hipEvent_t mid, void *vout, uint32_t *err) {
     constexpr uint64_t TILE = (uint64_t)BLOCK * IPT;
     // the 4-way tile holds 2 * TILE items (keys, and values for KV) in LDS
     constexpr uint64_t lds = 2 * TILE * (sizeof(K) + (HV ? sizeof(uint64_t) : 0));
     if constexpr (lds > 160 * 1024) {
-        (void)in, (void)out, (void)n, (void)R, (void)cout, (void)tab, (void)quad_lanes, (void)s, (void)mid, (void)vout;
+        (void)in, (void)out, (void)n, (void)R, (void)cout, (void)tab, (void)quad_lanes, (void)s, (void)mid, (void)vout,
+            (void)err;
         return hipErrorInvalidConfiguration;  // this merge shape has no 4-way form
     } else {
         if (n == 0) return hipSuccess;
@@ -877,7 +847,7 @@ static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R,
                                n, R, TILE / 2, nslots, tab);
         if (mid) (void)hipEventRecord(mid, s);
         hipLaunchKernelGGL((k_quad_merge_pass<K, HV, BLOCK, IPT>), dim3((unsigned)nwg), dim3(BLOCK), 0, s, in, out, n, R,
-                           cout, (const uint64_t *)tab, vout);
+                           cout, (const uint64_t *)tab, vout, err);
         return hipGetLastError();
     }
 }

@@ -9,6 +9,12 @@ namespace ogs {
 // Key classes: the kernels only see ordered unsigned keys.
 enum KeyClass { KC_32 = 0, KC_64 = 1, KC_KV = 2 };
 
+// Bits of a context's device error word (ogs_device_errors): set by a merge
+// kernel that finds its tile split table inconsistent and leaves that tile
+// unmerged.  Neither can happen with the tables the split kernels write.
+static constexpr uint32_t kErrKwaySplit = 1u;  // K-way pass
+static constexpr uint32_t kErrQuadSplit = 2u;  // 4-way pass
+
 // Batched-sort descriptors (layouts shared with the device structs).
 // Tile sort: sort `cnt & 0x3FFFFFFF` items from `in` to `out`; bit 30 of cnt
 // = final output of its segment (undo the key codec).
@@ -91,9 +97,9 @@ struct BlockShapeFns {
 };
 // One 4-way pass (runs of R -> 4R, tile/2 dividing R): the tile split table
 // tab (4 * quad_table_slots entries) is filled, then the merge runs; `mid` is
-// recorded between the two.
+// recorded between the two; `err` is the context's device error word.
 typedef hipError_t (*QuadLaunch)(const void *in, void *out, uint64_t n, uint64_t R, int cout, uint64_t *tab,
-                                 int quad_lanes, hipStream_t s, hipEvent_t mid, void *vout);
+                                 int quad_lanes, hipStream_t s, hipEvent_t mid, void *vout, uint32_t *err);
 // Tile slots of one 4-way pass (groups of 4R, 1 + 8R/tile slots each).
 inline uint64_t quad_table_slots(uint64_t n, uint64_t R, uint64_t tile) {
     return (n + 4 * R - 1) / (4 * R) * (1 + 8 * R / tile);
@@ -108,8 +114,10 @@ struct MergeShapeFns {
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
 
-// Counter-based input generator (gen_kernels.hip): width 4, 8 or 16 (KV);
+// Counter-based input generator (gen_kernels.hip): width = the item layout, 4,
+// 8 or 16 (u64 KV) bytes, or kGenKv32 for 8-byte {u32 key, u32 value} records;
 // cdf: the Zipf table (dist 1), starts/nb: Pareto burst starts (dist 9).
+static constexpr int kGenKv32 = 12;
 hipError_t launch_generate(int width, int dist, uint64_t seed, uint64_t n, void *out, const double *cdf,
                            const uint64_t *starts, uint64_t nb, hipStream_t s);
 // Shuffled bursts: (tag, burst key) pairs, then, once sorted by tag, the keys.
@@ -122,10 +130,10 @@ hipError_t launch_generate_untag(int width, const void *pairs, uint64_t n, void 
 // samples sm (both spaced 2^slog2) and fills tab (kway_tiles * kw entries),
 // then the merge kernel writes the output and, when `next` names a buffer, the
 // samples of the output runs for the pass after this one.  `mid` is recorded
-// between the two kernels.
+// between the two kernels; `err` is the context's device error word.
 typedef hipError_t (*KwayLaunch)(const void *in, void *out, uint64_t n, int rlog2, int slog2, int cout,
                                  uint64_t *tab, const void *s0, const void *sm, const SampleSpec *next, hipStream_t s,
-                                 hipEvent_t mid);
+                                 hipEvent_t mid, uint32_t *err);
 struct KwayShapeFns {
     int block, ipt;
     KwayLaunch run[5];  // fan-in 2, 4, 8, 16, 32

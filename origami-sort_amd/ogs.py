@@ -7,28 +7,31 @@ there is no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("OGS_LIB") or os.path.join(HERE, "libogs.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "ogs.h")
-DEFAULT_TILES = os.path.join(HERE, "ogs_tiles.txt")  # the tile table every new context loads
 
-U32, I32, U64, I64, F32, KV_U64U64, KV_I64I64 = range(7)
+U32, I32, U64, I64, F32, KV_U64U64, KV_I64I64, KV_U32U32 = range(8)
 KINDS = {"u32": U32, "i32": I32, "u64": U64, "i64": I64, "f32": F32,
-         "kv_u64u64": KV_U64U64, "kv_i64i64": KV_I64I64}
+         "kv_u64u64": KV_U64U64, "kv_i64i64": KV_I64I64, "kv_u32u32": KV_U32U32}
 KIND_NAMES = {v: k for k, v in KINDS.items()}
-WIDTH = {U32: 4, I32: 4, F32: 4, U64: 8, I64: 8, KV_U64U64: 16, KV_I64I64: 16}
+WIDTH = {U32: 4, I32: 4, F32: 4, U64: 8, I64: 8, KV_U64U64: 16, KV_I64I64: 16, KV_U32U32: 8}
+KV_KINDS = (KV_U64U64, KV_I64I64, KV_U32U32)
 KV_DTYPE = np.dtype([("key", "<u8"), ("value", "<u8")])
 KV_I_DTYPE = np.dtype([("key", "<i8"), ("value", "<i8")])
+KV32_DTYPE = np.dtype([("key", "<u4"), ("value", "<u4")])
 NP_DTYPE = {U32: np.dtype("<u4"), I32: np.dtype("<i4"), U64: np.dtype("<u8"), I64: np.dtype("<i8"),
-            F32: np.dtype("<f4"), KV_U64U64: KV_DTYPE, KV_I64I64: KV_I_DTYPE}
+            F32: np.dtype("<f4"), KV_U64U64: KV_DTYPE, KV_I64I64: KV_I_DTYPE, KV_U32U32: KV32_DTYPE}
 UNIFORM, ZIPF, SORTED, REVERSE, DISTINCT16, NORMAL = range(6)
 SAME, ALMOST_SORTED, PARETO, PARETO_BURSTS, PARETO_SHUFFLED, FIBONACCI = range(6, 12)
 GEN_COUNTER = 0x100  # the counter-based family for dists 0-5 (6-11 always are)
 
 _lib = None
+_live_contexts = weakref.WeakSet()
 
 
 class OgsError(RuntimeError):
@@ -52,6 +55,8 @@ def lib():
         vp, u64, i, cp = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, ctypes.c_char_p
         L.ogs_ctx_create.argtypes = [ctypes.POINTER(vp), i]
         L.ogs_ctx_destroy.argtypes = [vp]
+        L.ogs_device_errors.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32)]
+        L.ogs_tune_reset.argtypes = [vp]
         L.ogs_item_size.argtypes = [i]
         L.ogs_item_size.restype = ctypes.c_size_t
         L.ogs_sort.argtypes = [vp, i, vp, vp, u64, ctypes.POINTER(vp), vp]
@@ -208,6 +213,14 @@ class Context:
     def __init__(self, device=0):
         self.h = ctypes.c_void_p()
         check(lib().ogs_ctx_create(ctypes.byref(self.h), device))
+        _live_contexts.add(self)
+
+    def device_errors(self):
+        """ogs_device_errors: waits for the device and returns (and clears) the
+        context's error bits (1: K-way split table, 2: 4-way split table)."""
+        f = ctypes.c_uint32()
+        check(lib().ogs_device_errors(self.h, ctypes.byref(f)))
+        return f.value
 
     def close(self):
         if self.h:
@@ -301,6 +314,10 @@ class Context:
     def tune_load(self, path):
         check(lib().ogs_tune_load(self.h, path.encode()))
 
+    def tune_reset(self):
+        """Every key class back to the compiled-in tile table."""
+        check(lib().ogs_tune_reset(self.h))
+
     def tune_get(self, kind, param):
         v = ctypes.c_int()
         check(lib().ogs_tune_get(self.h, kind, param.encode(), ctypes.byref(v)))
@@ -324,6 +341,11 @@ def to_device(arr, device="cuda"):
 
 def from_device(t, kind, n):
     return t[: n * WIDTH[kind]].cpu().numpy().view(NP_DTYPE[kind]).copy()
+
+
+def live_contexts():
+    """The contexts this process has open."""
+    return [c for c in list(_live_contexts) if c.h]
 
 
 def declared_symbols():

@@ -52,6 +52,24 @@ int main(int argc, char **argv) {
         }
         fails += report("sort_single_thread<KeyValue<i64,i64>>", ok);
     }
+    {   // KeyValue<ui, ui>, the reference's 8-byte record (commons.h:71)
+        typedef KeyValue<ui, ui> KV;
+        static_assert(sizeof(KV) == 8, "packed 8-byte record");
+        std::vector<KV> d(n), orig;
+        std::mt19937 g(11);
+        for (ui64 i = 0; i < n; i++) { d[i].key = (ui)(g() % 5000); d[i].value = (ui)i; }
+        orig = d;
+        KV *o = origami_sorter::sort_multi_thread<KV, KV>(d.data(), nullptr, n, 8, 8, 4, nullptr);
+        bool ok = o != nullptr;
+        std::vector<char> seen(n, 0);
+        for (ui64 i = 0; ok && i < n; i++) {
+            if (i && o[i].key < o[i - 1].key) ok = false;
+            ui64 v = (ui64)o[i].value;
+            if (v >= n || seen[v] || orig[v].key != o[i].key) ok = false;
+            else seen[v] = 1;
+        }
+        fails += report("sort_multi_thread<KeyValue<ui,ui>>", ok);
+    }
     {   // sort_n < n: every chunk sorted independently (sorter.h:3857-3868)
         const ui64 chunk = n / 8 ? n / 8 : 1;
         std::vector<ui> d(n), tmp(n);

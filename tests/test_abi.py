@@ -20,7 +20,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_item_sizes():
     L = ogs.lib()
-    assert [L.ogs_item_size(k) for k in range(7)] == [4, 4, 8, 8, 4, 16, 16]
+    assert [L.ogs_item_size(k) for k in range(8)] == [4, 4, 8, 8, 4, 16, 16, 8]
     assert L.ogs_item_size(99) == 0
 
 
@@ -36,7 +36,7 @@ def test_bad_arguments_return_status_not_exit():
 
 
 @pytest.mark.parametrize("kind", [oracle.U32, oracle.I32, oracle.F32, oracle.U64, oracle.I64,
-                                  oracle.KV_U64U64, oracle.KV_I64I64])
+                                  oracle.KV_U64U64, oracle.KV_I64I64, oracle.KV_U32U32])
 @pytest.mark.parametrize("dist", [0, 1, 2, 3, 4, 5])
 def test_generator_matches_oracle(kind, dist):
     if dist == 5 and kind != oracle.F32:

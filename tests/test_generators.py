@@ -16,7 +16,7 @@ import pytest
 import ogs
 import oracle
 
-KINDS = range(7)
+KINDS = range(8)
 DISTS = range(12)
 SIZES = [0, 1, 2, 7, 94, 95, 1000, 4099, 65537]
 

@@ -39,7 +39,7 @@ def configure(ctx, kind, fanin, sample=5, shape=0, small_block=True):
 
 
 def restore(ctx, kind):
-    ctx.tune_load(ogs.DEFAULT_TILES)
+    ctx.tune_reset()
 
 
 def gpu_sort(ctx, kind, a):
@@ -216,7 +216,7 @@ def test_kway_default_tiles(ctx, kind, dist):
     # so the split kernel takes its search path, uniform keys its sample window
     n = (3 << 20) + 777 if kind != ogs.KV_U64U64 else (3 << 19) + 555
     say(f"case kway default tiles kind={ogs.KIND_NAMES[kind]} dist={dist} n={n}")
-    ctx.tune_load(ogs.DEFAULT_TILES)
+    ctx.tune_reset()
     assert ctx.tune_get(kind, "kway_fanin") > 0
     a = ogs.generate(kind, dist, n, seed=77 + dist)
     if kind == ogs.KV_U64U64:

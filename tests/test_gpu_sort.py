@@ -12,7 +12,8 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-KV = (ogs.KV_U64U64, ogs.KV_I64I64)
+KV = (ogs.KV_U64U64, ogs.KV_I64I64, ogs.KV_U32U32)
+KV16 = (ogs.KV_U64U64, ogs.KV_I64I64)  # the kinds ogs_sort_soa takes
 
 
 @pytest.fixture(scope="module")
@@ -73,7 +74,8 @@ def test_u32_config1_2pow20_seed42(ctx):
     assert_same(ogs.U32, got, a)
 
 
-@pytest.mark.parametrize("kind", [ogs.U32, ogs.I32, ogs.F32, ogs.U64, ogs.I64, ogs.KV_U64U64, ogs.KV_I64I64])
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.I32, ogs.F32, ogs.U64, ogs.I64, ogs.KV_U64U64, ogs.KV_I64I64,
+                                  ogs.KV_U32U32])
 @pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.ZIPF, ogs.SORTED, ogs.REVERSE, ogs.DISTINCT16])
 @pytest.mark.parametrize("n", [777, 5000, 300007])
 def test_kinds_and_distributions(ctx, kind, dist, n):
@@ -82,7 +84,7 @@ def test_kinds_and_distributions(ctx, kind, dist, n):
     assert_same(kind, gpu_sort(ctx, kind, a), a)
 
 
-@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.I32, ogs.I64, ogs.KV_I64I64])
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.I32, ogs.I64, ogs.KV_I64I64, ogs.KV_U32U32])
 @pytest.mark.parametrize("n", [5, 300, 3001, 70001])
 def test_extreme_keys(ctx, kind, n):
     # maximum and minimum keys mixed with pads in partial tiles
@@ -92,7 +94,7 @@ def test_extreme_keys(ctx, kind, n):
     raw = a.view(np.uint8).reshape(n, -1)
     pick = rng.random(n)
     w = ogs.WIDTH[kind]
-    kw = min(w, 8)
+    kw = 4 if kind == ogs.KV_U32U32 else min(w, 8)
     raw[pick < 0.2, :kw] = 0xFF
     raw[(pick >= 0.2) & (pick < 0.3), :kw] = 0x00
     if kind in (ogs.I32, ogs.I64, ogs.KV_I64I64):
@@ -100,6 +102,22 @@ def test_extreme_keys(ctx, kind, n):
         raw[(pick >= 0.3) & (pick < 0.4), : kw - 1] = 0
         raw[(pick >= 0.4) & (pick < 0.5), kw - 1] = 0x7F
     assert_same(kind, gpu_sort(ctx, kind, a), a)
+
+
+@pytest.mark.parametrize("n", [1000, 70001, (1 << 20) + 5, 3 << 20])
+def test_kv32_records(ctx, n):
+    # KeyValue<ui, ui>: 8-byte records sorted on the key (the oracle's bar: key
+    # sequence bit-exact, each key's values the same multiset); this library
+    # orders equal keys by value, so the output is also the lexicographic
+    # (key, value) order exactly
+    say(f"case kv32 n={n}")
+    rng = np.random.default_rng(n)
+    a = np.empty(n, dtype=ogs.NP_DTYPE[ogs.KV_U32U32])
+    a["key"] = rng.integers(0, max(2, n // 8), n, dtype=np.uint32)
+    a["value"] = rng.permutation(n).astype(np.uint32)
+    got = gpu_sort(ctx, ogs.KV_U32U32, a)
+    assert_same(ogs.KV_U32U32, got, a)
+    assert np.array_equal(got.view(np.uint8), a[np.lexsort((a["value"], a["key"]))].view(np.uint8))
 
 
 def test_f32_specials(ctx):
@@ -111,7 +129,7 @@ def test_f32_specials(ctx):
     assert_same(ogs.F32, gpu_sort(ctx, ogs.F32, a), a)
 
 
-@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32])
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32, ogs.KV_U32U32])
 def test_chunked_sort(ctx, kind):
     import torch
     n, chunk = 300000, 4096 * 3
@@ -129,7 +147,7 @@ def test_chunked_sort(ctx, kind):
         assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
 
 
-@pytest.mark.parametrize("kind", [ogs.U32, ogs.I64, ogs.KV_U64U64, ogs.F32])
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.I64, ogs.KV_U64U64, ogs.F32, ogs.KV_U32U32])
 def test_segmented_sort(ctx, kind):
     import torch
     say(f"case segmented kind={ogs.KIND_NAMES[kind]}")
@@ -211,7 +229,7 @@ def test_host_entry_point(ctx):
     assert np.array_equal(got, np.sort(a))
 
 
-@pytest.mark.parametrize("kind", KV)
+@pytest.mark.parametrize("kind", KV16)
 @pytest.mark.parametrize("n", [0, 1, 2, 1000, 100003, (1 << 20) + 5])
 def test_structure_of_arrays_kv(ctx, kind, n):
     # ogs_sort_soa: keys and values in separate arrays, packed, sorted and
@@ -264,6 +282,9 @@ def test_every_compiled_shape(ctx, kind):
                 say(f"case shape kind={ogs.KIND_NAMES[kind]} block={bshapes[bi]} merge={mshapes[mi]} n={n}")
                 a = ogs.generate(kind, ogs.UNIFORM if n % 2 else ogs.DISTINCT16, n, seed=n)
                 assert_same(kind, gpu_sort(ctx, kind, a), a)
+        # the 2-way (and, where the table asks for it, 4-way) merge passes of
+        # every merge shape: the K-way plan off
+        ctx.tune_set(kind, "kway_fanin", 0)
         for mi, (b, ipt, w) in enumerate(mshapes):
             for corank in (0, 1):
                 ctx.tune_set(kind, "plain_block", 0)
@@ -274,9 +295,7 @@ def test_every_compiled_shape(ctx, kind):
                 a = ogs.generate(kind, ogs.ZIPF, n, seed=mi)
                 assert_same(kind, gpu_sort(ctx, kind, a), a)
     finally:
-        ctx.tune_set(kind, "plain_block", 0)
-        ctx.tune_set(kind, "plain_merge", 0)
-        ctx.tune_set(kind, "plain_corank", 0)
+        ctx.tune_reset()
 
 
 @pytest.mark.parametrize("plain_min", [22, 16])
@@ -309,7 +328,7 @@ def test_pointer_batch(ctx, kind, plain_min):
         ctx.sort_batch_ptrs(kind, items)
         torch.cuda.synchronize()
     finally:
-        ctx.tune_set(kind, "seg_plain_min", 22)
+        ctx.tune_reset()
     for a, (din, out, scr), n in zip(arrays, keep, lens):
         if n == 0:
             continue
@@ -340,7 +359,7 @@ def test_segmented_tiny_class(ctx):
         ref = oracle.segmented_sort(kind, a, off, threads=4)
         assert np.array_equal(got, ref)
     finally:
-        ctx.tune_set(kind, "seg_tiny_max", 0)
+        ctx.tune_reset()
 
 
 @pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32, ogs.I64])

@@ -57,12 +57,26 @@ def split(X, v, up, slog2, samples_j):
     return a
 
 
+def bounds_window(kw):
+    """The merged samples k_kway_bounds reads per tile: LPT lanes per tile (32
+    when two tiles share a wave, fan-in <= 8; else 64), NW = max(1, 8 * kw / LPT)
+    samples per lane (kw_bounds_tpw and NW in kway_kernels.hip)."""
+    lpt = 32 if kw <= 8 else 64
+    return lpt * max(1, 8 * kw // lpt)
+
+
+def test_bounds_window_matches_kernel_constants():
+    # fan-in 2 and 4 read 32 samples (one per lane of a half wave), 8 reads
+    # 64, 16 reads 128, 32 reads 256
+    assert [bounds_window(k) for k in (2, 4, 8, 16, 32)] == [32, 32, 64, 128, 256]
+
+
 def split_window(X, v, r, j, slog2, samples_j, tags, p, kw):
     """k_kway_bounds as the kernel runs it: the count of run j's samples that
     precede the boundary (merged position p) is the index of run j's first
-    sample in the window of 8 * kw merged samples from p; with none there, all
-    of them when the window reaches the group's end, else the binary search."""
-    kw_window = max(64, 8 * kw)
+    sample in the window of merged samples from p; with none there, all of them
+    when the window reaches the group's end, else the binary search."""
+    kw_window = bounds_window(kw)
     window = tags[p:p + kw_window]
     lo = None
     for (_, rj, mj) in window:
@@ -456,3 +470,24 @@ def test_group_search_matches_bisect(G):
             # (G+1)-ary: about log_{G+1} of the range
             if hi > lo:
                 assert steps <= int(np.ceil(np.log(hi - lo + 1) / np.log(G + 1))) + 2
+
+
+@pytest.mark.parametrize("kw", [2, 4, 8])
+def test_window_and_search_paths_both_taken(kw):
+    # with the kernel's window (32 merged samples for fan-in 2 and 4) uniform
+    # runs resolve in the window and sorted runs, which do not interleave,
+    # need the search: both paths of k_kway_bounds are modelled at each fan-in
+    block, ipt, slog2 = 64, 8, 3
+    R = 1024
+    n = R * kw * 2 + 3 * R + 77
+    taken = {}
+    for name, keys in (("uniform", np.random.default_rng(kw).integers(0, 2**32, n, dtype=np.uint64)),
+                       ("sorted", np.arange(n, dtype=np.uint64) // 3)):
+        PATHS.clear()
+        keys = runs_of(keys, R)
+        out = model_pass(keys, R, kw, slog2, block, ipt)
+        ref = np.concatenate([np.sort(keys[b:b + kw * R]) for b in range(0, n, kw * R)])
+        assert np.array_equal(out, ref)
+        taken[name] = dict(PATHS)
+    assert taken["uniform"].get("window", 0) > 0
+    assert taken["sorted"].get("search", 0) > 0

@@ -23,7 +23,7 @@ def ordered(kind, a):
         return a.view(np.uint32) ^ np.uint32(0x80000000)
     if kind == oracle.I64:
         return a.view(np.uint64) ^ np.uint64(1 << 63)
-    if kind in (oracle.KV_U64U64,):
+    if kind in (oracle.KV_U64U64, oracle.KV_U32U32):
         return a["key"]
     if kind == oracle.KV_I64I64:
         return a["key"].view(np.uint64) ^ np.uint64(1 << 63)
@@ -143,7 +143,9 @@ def test_threaded_sort_of_bursty_input(dist):
         assert np.array_equal(oracle.sort(oracle.U32, a, threads=threads, min_k=min_k), np.sort(a))
 
 
-KINDS =[oracle.U32, oracle.I32, oracle.F32, oracle.U64, oracle.I64, oracle.KV_U64U64, oracle.KV_I64I64]
+KINDS = [oracle.U32, oracle.I32, oracle.F32, oracle.U64, oracle.I64, oracle.KV_U64U64, oracle.KV_I64I64,
+         oracle.KV_U32U32]
+KV_KINDS = (oracle.KV_U64U64, oracle.KV_I64I64, oracle.KV_U32U32)
 
 
 @pytest.mark.parametrize("kind", KINDS)
@@ -156,13 +158,33 @@ def test_sort_matches_numpy(kind, n, threads, min_k):
         a = a.view(a.dtype)  # full range: negative keys present
     s = oracle.sort(kind, a, threads=threads, min_k=min_k)
     ref = numpy_sorted(kind, a)
-    if kind in (oracle.KV_U64U64, oracle.KV_I64I64):
+    if kind in KV_KINDS:
         assert np.array_equal(s["key"], ref["key"])
         # values = original index: a permutation, each pointing at its own key
-        assert np.array_equal(np.sort(s["value"]), np.arange(n, dtype=np.uint64))
+        assert np.array_equal(np.sort(s["value"]), np.arange(n, dtype=s["value"].dtype))
         assert np.array_equal(a["key"][s["value"].astype(np.int64)], s["key"])
     else:
         assert np.array_equal(s.view(np.uint8), ref.view(np.uint8))
+
+
+def test_kv32_sorts_on_the_key_alone():
+    # KeyValue<ui, ui> compares keys only (Origami/commons.h:55-57): with the
+    # values in descending order under every key, a key-only sort must still
+    # put the keys in order and keep each key's values (a sort on the whole
+    # 8-byte record would also pass; this pins the key's position in the record)
+    n = 50000
+    a = np.empty(n, dtype=oracle.DTYPES[oracle.KV_U32U32])
+    rng = np.random.default_rng(3)
+    a["key"] = rng.integers(0, 300, n, dtype=np.uint32)
+    a["value"] = np.arange(n, 0, -1, dtype=np.uint32)
+    s = oracle.sort(oracle.KV_U32U32, a, threads=4, min_k=4)
+    assert np.array_equal(s["key"], np.sort(a["key"]))
+    order_in = np.lexsort((a["value"], a["key"]))
+    order_out = np.lexsort((s["value"], s["key"]))
+    assert np.array_equal(a[order_in].view(np.uint8), s[order_out].view(np.uint8))
+    # the key sits in the low four bytes: sorting the raw 8-byte words as u64
+    # would order by the value instead
+    assert not np.array_equal(np.sort(a.view(np.uint64)).view(a.dtype)["key"], s["key"])
 
 
 @pytest.mark.parametrize("dist", [oracle.ZIPF, oracle.SORTED, oracle.REVERSE, oracle.DISTINCT16])
