@@ -96,6 +96,21 @@ size_t ogs_item_size(ogs_key_t kt);
 int ogs_sort(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch, uint64_t n,
              void **d_sorted, void *stream);
 
+/* ogs_sort without allocation: the same sort, with its device workspace (the
+ * K-way pass's samples and split tables, the merge passes' co-rank tables)
+ * taken from a static device array of OGS_NOALLOC_SCRATCH_BYTES (32 MiB) that
+ * each GPU holds for the life of the process, instead of the context's
+ * per-stream pool, which grows with hipMalloc.  No reference counterpart: it
+ * serves callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The call makes no hipMalloc or hipFree and
+ * records no profiling events.  Sizes whose workspace exceeds the array fail
+ * with OGS_ERR_ARG before anything is launched: above about 2^26 keys for
+ * 4-byte kinds, 2^24 for 8-byte kinds and 2^23 pairs for 16-byte KV with the
+ * default tile table.  Calls that overlap on one GPU share the array and race. */
+#define OGS_NOALLOC_SCRATCH_BYTES (32u << 20)
+int ogs_sort_noalloc(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch, uint64_t n,
+                     void **d_sorted, void *stream);
+
 /* Out-of-place form: reads n items from d_in (left intact) and sorts them
  * using d_data and d_scratch (n items each) as ping-pong buffers; the result
  * lands in *d_sorted (d_data or d_scratch).  The reference always sorts in

@@ -8,7 +8,7 @@ FLAGS="--offload-arch=gfx950 -O3 -std=c++17 -fPIC -Wall -Wno-unused-function -Xc
 mkdir -p _build
 rm -f _build/*.o
 pids=()
-for src in csrc/sort_kernels.hip csrc/kway_kernels.hip csrc/gen_kernels.hip csrc/ogs_api.cpp csrc/ogs_host.cpp csrc/ogs_serve.cpp; do
+for src in csrc/sort_kernels.hip csrc/kway_kernels.hip csrc/gen_kernels.hip csrc/ogs_noalloc.hip csrc/ogs_api.cpp csrc/ogs_host.cpp csrc/ogs_serve.cpp; do
     obj="_build/$(basename "${src%.*}").o"
     $HIPCC $FLAGS "$@" -c "$src" -o "$obj" &
     pids+=($!)

@@ -242,7 +242,7 @@ void Profiler::reset() {
 
 // Plain sort.  Returns the buffer holding the result in *out.
 int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s,
-              const void *vin, void *kout, void *vout) {
+              const void *vin, void *kout, void *vout, const ExtScratch *ext) {
     KeyInfo ki;
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
     if ((vin || vout) && ki.width != 16) return fail(OGS_ERR_ARG, "separate value arrays need a key-value type");
@@ -259,7 +259,8 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     // before writing it back
     hipError_t e = hipSetDevice(device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-    if (!vin && !vout && kway_on(ki, n)) return sort_kway(ki, in, data, scratch, n, out, s);
+    if (!vin && !vout && kway_on(ki, n)) return sort_kway(ki, in, data, scratch, n, out, s, ext);
+    const bool prof_on = prof.on && !ext;  // profiling events are allocated
     const BlockShapeFns &bs = block_shape(ki.kc, TP_PLAIN_BLOCK);
     const MergeShapeFns &ms = merge_shape(ki.kc, TP_PLAIN_MERGE, bs.tile());
     const uint64_t T = bs.tile();
@@ -269,13 +270,13 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     // and the table asks for fan-in 4, else 2-way; start on the buffer that
     // makes the last pass land in `data` when reading from a separate input
     const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4 && ms.quad_ok;
-    std::vector<int> ways;
+    int ways[kMaxPasses];
+    int merges = 0;
     for (uint64_t R = T; R < n;) {
         const int wy = (quad && 2 * R < n) ? 4 : 2;
-        ways.push_back(wy);
+        ways[merges++] = wy;
         R *= (uint64_t)wy;
     }
-    const int merges = (int)ways.size();
     void *src = scratch, *dst = data;
     const void *bin = data;
     if (in) {
@@ -284,14 +285,14 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
         dst = (src == data) ? scratch : data;
     }
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    if (prof.on) {
+    if (prof_on) {
         ev[0] = prof.get();
         ev[1] = prof.get();
         (void)hipEventRecord(ev[0], s);
     }
     e = bs.run(bin, one && vout ? kout : src, n, ki.codec, one ? ki.codec : 0, s, vin, one ? vout : nullptr, nullptr);
     if (e != hipSuccess) return hip_fail(e, "block sort launch");
-    if (prof.on) {
+    if (prof_on) {
         (void)hipEventRecord(ev[1], s);
         prof.pending.push_back({PK_BLOCK, ev[0], ev[1], n, true});
     }
@@ -301,37 +302,44 @@ int Ctx::sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t 
     uint64_t *part = nullptr;
     const bool table = tune[ki.kc].v[TP_PLAIN_CORANK] == 0;
     if (!one && (table || quad)) {
-        Workspace *w = workspace(s);
-        std::lock_guard<std::mutex> g(w->mu);
         uint64_t need = (n + MT - 1) / MT + 4;
         uint64_t R = T;
-        for (int wy : ways) {
-            if (wy == 4) need = std::max<uint64_t>(need, 4 * quad_table_slots(n, R, MT) + 8);
-            R *= (uint64_t)wy;
+        for (int i = 0; i < merges; i++) {
+            if (ways[i] == 4) need = std::max<uint64_t>(need, 4 * quad_table_slots(n, R, MT) + 8);
+            R *= (uint64_t)ways[i];
         }
-        int rc = w->reserve_part(need * sizeof(uint64_t));
-        if (rc) return rc;
-        part = (uint64_t *)w->part;
+        if (ext) {
+            if (need * sizeof(uint64_t) > ext->cap)
+                return fail(OGS_ERR_ARG, "n=%llu needs %llu bytes of merge tables, above the %zu-byte scratch",
+                            (unsigned long long)n, (unsigned long long)(need * sizeof(uint64_t)), ext->cap);
+            part = (uint64_t *)ext->base;
+        } else {
+            Workspace *w = workspace(s);
+            std::lock_guard<std::mutex> g(w->mu);
+            int rc = w->reserve_part(need * sizeof(uint64_t));
+            if (rc) return rc;
+            part = (uint64_t *)w->part;
+        }
     }
     if (!one) {
         uint64_t R = T;
         for (int pi = 0; pi < merges; pi++) {
             const int wy = ways[pi];
             const bool last = pi == merges - 1;
-            if (prof.on) {
+            if (prof_on) {
                 for (int k = 0; k < 3; k++) ev[k] = prof.get();
                 (void)hipEventRecord(ev[0], s);
             }
             if (wy == 4)
                 e = ms.run_quad(src, last && vout ? kout : dst, n, R, last ? ki.codec : 0, part,
-                                tune[ki.kc].v[TP_QUAD_LANES], s, prof.on ? ev[1] : nullptr, last ? vout : nullptr,
+                                tune[ki.kc].v[TP_QUAD_LANES], s, prof_on ? ev[1] : nullptr, last ? vout : nullptr,
                                 err);
             else
                 e = ms.run(src, last && vout ? kout : dst, n, R, last ? ki.codec : 0, table ? part : nullptr,
-                           tune[ki.kc].v[TP_PARTITION_LANES], s, prof.on ? ev[1] : nullptr, last ? vout : nullptr);
+                           tune[ki.kc].v[TP_PARTITION_LANES], s, prof_on ? ev[1] : nullptr, last ? vout : nullptr);
             R *= (uint64_t)wy;
             if (e != hipSuccess) return hip_fail(e, "merge pass launch");
-            if (prof.on) {
+            if (prof_on) {
                 (void)hipEventRecord(ev[2], s);
                 // ev[1] closes the partition and opens the merge; the
                 // partition record owns it
@@ -354,14 +362,13 @@ bool Ctx::kway_on(const KeyInfo &ki, uint64_t n) {
 
 // Merge levels per K-way pass: ceil(levels / log2(fan-in)) passes, the levels
 // spread evenly over them, the larger passes first.
-std::vector<int> Ctx::kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile) {
+int Ctx::kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile, int *lv) {
     int L = 0;
     for (uint64_t R = block_tile; R < n; R <<= 1) L++;
     const int lmax = ilog2((uint64_t)tune[kc].v[TP_KWAY_FANIN]);
     const int P = (L + lmax - 1) / lmax;
-    std::vector<int> lv(P);
     for (int i = 0; i < P; i++) lv[i] = L / P + (i < L % P ? 1 : 0);
-    return lv;
+    return P;
 }
 
 // Plain sort with sample-split K-way passes.  Pass p merges groups of
@@ -370,7 +377,7 @@ std::vector<int> Ctx::kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile) {
 // group by 2-way merge passes over the sample array.  Sample buffers: S0 holds
 // the unmerged samples of the current input; S1 and S2 take the sample merges.
 int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch, uint64_t n, void **out,
-                   hipStream_t s) {
+                   hipStream_t s, const ExtScratch *ext) {
     const BlockShapeFns &bs = block_shape(ki.kc, TP_PLAIN_BLOCK);
     int kcnt = 0;
     const KwayShapeFns *kt = kway_shapes(ki.kc, &kcnt);
@@ -379,11 +386,11 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
     const KwayShapeFns &ks = kt[kidx];
     const uint64_t T = bs.tile();
     if (T & (T - 1)) return fail(OGS_ERR_ARG, "K-way passes need a power-of-two block tile");
-    const std::vector<int> lv = kway_plan(ki.kc, n, T);
-    const int P = (int)lv.size();
+    int lv[kMaxPasses], sl[kMaxPasses];
+    const int P = kway_plan(ki.kc, n, T, lv);
+    const bool prof_on = prof.on && !ext;  // profiling events are allocated
     // sample spacing per pass: at most the tuned one, and at most R / 256 so
     // the smallest 2-way sample merge tile (512) divides every pair span
-    std::vector<int> sl(P);
     int rl = ilog2(T), smin = 63;
     for (int p = 0; p < P; p++) {
         sl[p] = std::min(tune[ki.kc].v[TP_KWAY_SAMPLE], rl - 8);
@@ -409,11 +416,20 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
     auto al = [](uint64_t b) { return (b + 255) & ~(uint64_t)255; };
     const uint64_t o_s1 = al(ns_max * sw), o_s2 = o_s1 + al(ns_max * sw), o_tab = o_s2 + al(ns_max * sw),
                    o_part = o_tab + al(tab_n * 8), bytes = o_part + al(part_n * 8);
-    Workspace *w = workspace(s);
-    std::lock_guard<std::mutex> g(w->mu);
-    int rc = w->reserve(bytes);
-    if (rc) return rc;
-    char *D = (char *)w->dev;
+    char *D = nullptr;
+    std::unique_lock<std::mutex> g;
+    if (ext) {
+        if (bytes > ext->cap)
+            return fail(OGS_ERR_ARG, "n=%llu needs %llu bytes of K-way samples and split tables, above the %zu-byte "
+                        "scratch", (unsigned long long)n, (unsigned long long)bytes, ext->cap);
+        D = ext->base;
+    } else {
+        Workspace *w = workspace(s);
+        g = std::unique_lock<std::mutex>(w->mu);
+        int rc = w->reserve(bytes);
+        if (rc) return rc;
+        D = (char *)w->dev;
+    }
     void *S0 = D, *S1 = D + o_s1, *S2 = D + o_s2;
     uint64_t *tab = (uint64_t *)(D + o_tab), *mpart = (uint64_t *)(D + o_part);
     void *src = scratch, *dst = data;
@@ -424,7 +440,7 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
         dst = (src == data) ? scratch : data;
     }
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-    if (prof.on) {
+    if (prof_on) {
         ev[0] = prof.get();
         ev[1] = prof.get();
         (void)hipEventRecord(ev[0], s);
@@ -433,7 +449,7 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
     const SampleSpec first = {S0, rl, 1 << lv[0], sl[0]};
     hipError_t e = bs.run(bin, src, n, ki.codec, 0, s, nullptr, nullptr, &first);
     if (e != hipSuccess) return hip_fail(e, "block sort launch");
-    if (prof.on) {
+    if (prof_on) {
         (void)hipEventRecord(ev[1], s);
         prof.pending.push_back({PK_BLOCK, ev[0], ev[1], n, true});
     }
@@ -442,7 +458,7 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
     const int lanes = tune[skc].v[TP_PARTITION_LANES];
     for (int p = 0; p < P; p++) {
         const bool last = p == P - 1;
-        if (prof.on) {
+        if (prof_on) {
             for (int k = 0; k < 3; k++) ev[k] = prof.get();
             (void)hipEventRecord(ev[0], s);
         }
@@ -480,9 +496,9 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
         SampleSpec next = {nullptr, 0, 0, 0};
         if (!last) next = {S0, rl + lv[p], 1 << lv[p + 1], sl[p + 1]};
         e = ks.run[lv[p] - 1](src, dst, n, rl, sl[p], last ? ki.codec : 0, tab, S0, sa, last ? nullptr : &next, s,
-                              prof.on ? ev[1] : nullptr, err);
+                              prof_on ? ev[1] : nullptr, err);
         if (e != hipSuccess) return hip_fail(e, "K-way pass launch");
-        if (prof.on) {
+        if (prof_on) {
             (void)hipEventRecord(ev[2], s);
             prof.pending.push_back({PK_PARTITION, ev[0], ev[1], n, true});
             prof.pending.push_back({PK_KWAY, ev[1], ev[2], n, false});
@@ -498,7 +514,8 @@ int Ctx::pass_count(ogs_key_t kt, uint64_t n) {
     KeyInfo ki;
     if (!key_info(kt, &ki)) return fail(OGS_ERR_ARG, "unknown key type %d", (int)kt);
     if (n == 0) return 0;
-    if (kway_on(ki, n)) return 1 + (int)kway_plan(ki.kc, n, block_shape(ki.kc, TP_PLAIN_BLOCK).tile()).size();
+    int lv[kMaxPasses];
+    if (kway_on(ki, n)) return 1 + kway_plan(ki.kc, n, block_shape(ki.kc, TP_PLAIN_BLOCK).tile(), lv);
     const uint64_t T = block_shape(ki.kc, TP_PLAIN_BLOCK).tile();
     const bool quad = tune[ki.kc].v[TP_PLAIN_FANIN] == 4 && merge_shape(ki.kc, TP_PLAIN_MERGE, T).quad_ok;
     int p = 1;
@@ -747,6 +764,11 @@ int ogs_ctx_create(ogs_ctx **ctx, int device) {
     if (e != hipSuccess) {
         if (err) (void)hipFree(err);
         return fail(OGS_ERR_NOMEM, "device error word: %s", hipGetErrorString(e));
+    }
+    const int nrc = noalloc_init(device);
+    if (nrc) {
+        (void)hipFree(err);
+        return nrc;
     }
     Ctx *c = new Ctx();
     c->device = device;

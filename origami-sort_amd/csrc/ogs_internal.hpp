@@ -127,6 +127,14 @@ struct BatchBuild {
     void pack(uint8_t *h) const;  // bytes [o_seg, bytes) of the table, into h[0 ..)
 };
 
+// Caller-provided device workspace for a plain sort (ogs_sort_noalloc): the
+// co-rank tables, K-way samples and split tables are carved from it instead of
+// the stream's pooled workspace, and nothing is allocated.
+struct ExtScratch {
+    char *base;
+    size_t cap;
+};
+
 struct Ctx {
     int device = 0;
     uint32_t *err = nullptr;  // device error word (kErr* bits), read and cleared by ogs_device_errors
@@ -145,12 +153,16 @@ struct Ctx {
     // the input from `in` (left intact) and ping-pong between data and scratch.
     // vin: KV values apart from the keys in `in`; kout/vout: the sorted keys and
     // values written to separate arrays by the last pass (structure of arrays)
+    // ext: the workspace comes from ext (no allocation, no profiling events)
     int sort(ogs_key_t kt, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s,
-             const void *vin = nullptr, void *kout = nullptr, void *vout = nullptr);
+             const void *vin = nullptr, void *kout = nullptr, void *vout = nullptr, const ExtScratch *ext = nullptr);
     // plain sort with sample-split K-way merge passes (n > block tile)
-    int sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s);
-    // merge passes of a K-way plain sort: levels per pass (largest first)
-    std::vector<int> kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile);
+    int sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch, uint64_t n, void **out, hipStream_t s,
+                  const ExtScratch *ext);
+    // merge passes of a K-way plain sort: fills lv[0 .. P) with the levels per
+    // pass (largest first) and returns P (at most kMaxPasses)
+    static constexpr int kMaxPasses = 64;
+    int kway_plan(KeyClass kc, uint64_t n, uint64_t block_tile, int *lv);
     bool kway_on(const KeyInfo &ki, uint64_t n);
     int segmented(ogs_key_t kt, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg, void **out,
                   hipStream_t s);
@@ -163,6 +175,10 @@ struct Ctx {
 };
 
 Tuning default_tuning(KeyClass kc);
+// The no-allocation scratch array's device address on `device` (the current
+// device), looked up once per device (ogs_noalloc.hip); called by
+// ogs_ctx_create.
+int noalloc_init(int device);
 // Applies the tile-table file named by $OGS_TILES, when set (all or nothing).
 int load_env_tiles(ogs_ctx *ctx);
 int csr_segments(const KeyInfo &ki, void *data, void *scratch, const uint64_t *h_off, uint64_t nseg,

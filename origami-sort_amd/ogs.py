@@ -62,6 +62,7 @@ def lib():
         L.ogs_sort.argtypes = [vp, i, vp, vp, u64, ctypes.POINTER(vp), vp]
         L.ogs_sort_chunks.argtypes = [vp, i, vp, vp, u64, u64, ctypes.POINTER(vp), vp]
         L.ogs_sort_keep.argtypes = [vp, i, vp, vp, vp, u64, ctypes.POINTER(vp), vp]
+        L.ogs_sort_noalloc.argtypes = [vp, i, vp, vp, u64, ctypes.POINTER(vp), vp]
         L.ogs_sort_soa.argtypes = [vp, i, vp, vp, vp, u64, vp]
         L.ogs_profile_enable.argtypes = [vp, i]
         L.ogs_profile_read.argtypes = [vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(u64),
@@ -238,6 +239,14 @@ class Context:
         out = ctypes.c_void_p()
         check(lib().ogs_sort(self.h, kind, ctypes.c_void_p(data), ctypes.c_void_p(scratch), n,
                              ctypes.byref(out), ctypes.c_void_p(stream)))
+        return out.value
+
+    def sort_noalloc_ptr(self, kind, data, scratch, n, stream=0):
+        """ogs_sort_noalloc: ogs_sort with its workspace in the library's static
+        device array (no allocation)."""
+        out = ctypes.c_void_p()
+        check(lib().ogs_sort_noalloc(self.h, kind, ctypes.c_void_p(data), ctypes.c_void_p(scratch), n,
+                                     ctypes.byref(out), ctypes.c_void_p(stream)))
         return out.value
 
     def sort_keep_ptr(self, kind, src, data, scratch, n, stream=0):
