@@ -293,6 +293,13 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
     __shared__ uint64_t s_o0;       // output index of the tile's first item
     __shared__ int s_cnt;           // items in the tile (-1: inconsistent table, reported in *err)
     __shared__ int s_top[LOGKW + 1];  // per level: co-rank step bound over all pairs
+    // key-only: the first and last item of every IPT-item block of the current
+    // runs (block k of the run at LDS offset o at index o / IPT + k, in the
+    // run's ascending order), for the block-granular co-rank steps; one guard
+    // entry either side (a step clamped to its current answer reads index -1
+    // or BLOCK, and its result does not depend on the value)
+    __shared__ K s_spFb[HV ? 1 : BLOCK + 2], s_spLb[HV ? 1 : BLOCK + 2];
+    K *s_spF = s_spFb + (HV ? 0 : 1), *s_spL = s_spLb + (HV ? 0 : 1);
     K *sk = skb + lds_front<K>();
     uint64_t *sv = svb + (HV ? lds_front<K>() : 0);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -480,6 +487,23 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
         }
     }
     __syncthreads();
+    if constexpr (!HV) {
+        // the slices' block splitters: an ascending slice's block is this
+        // thread's chunk; an odd slice is stored descending, so the chunk at
+        // c0 is its block (P[j0+1] - c0) / IPT - 1 read backwards
+        if (live) {
+            const K first = sk[lsw<K>(c0)], last = sk[lsw<K>(c0 + IPT - 1)];
+            if (j0 & 1) {
+                const int idx = (s_P[j0] + s_P[j0 + 1] - c0) / IPT - 1;
+                s_spF[idx] = last;
+                s_spL[idx] = first;
+            } else {
+                s_spF[tid] = first;
+                s_spL[tid] = last;
+            }
+        }
+        __syncthreads();
+    }
     K r[IPT];
     uint64_t v[IPT];
 #pragma unroll 1
@@ -493,7 +517,9 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
             diag = c0 - a_off;
             const int top = __builtin_amdgcn_readfirstlane(s_top[lev]);
             if constexpr (!HV) {
-                lds_merge_slice_rev<IPT>(sk, a_off, mid - a_off, end - 1, end - mid, diag, IPT, top, r);
+                const int a = lds_corank_split_rev<IPT>(sk, s_spF, s_spL, a_off, mid - a_off, mid, end - mid, end - 1,
+                                                        diag, top);
+                lds_merge_rev<IPT, true>(sk, a_off + a, end - 1 - (diag - a), 0, IPT, r);
             } else {
                 // KV: ascending runs; merges test exhaustion against the real
                 // lengths, so a pad never displaces a real maximum-key item and
@@ -519,6 +545,14 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
             // the next level's B sides are stored descending (key-only)
             if (!HV && lev < LOGKW && ((j0 >> lev) & 1)) lds_put_desc<IPT>(sk, end - 1 - diag, r);
             else lds_put<IPT, HV>(sk, sv, c0, r, v);
+            if constexpr (!HV) {
+                // this thread's outputs are block (c0 - pair start) / IPT of
+                // the merged run at the pair's offset: splitter index tid
+                if (lev < LOGKW) {
+                    s_spF[tid] = r[0];
+                    s_spL[tid] = r[IPT - 1];
+                }
+            }
         }
         __syncthreads();
     }

@@ -482,6 +482,49 @@ __device__ __forceinline__ void lds_merge_slice_rev(const K *sk, int a_off, int 
     else lds_merge_rev<IPT, false>(sk, a_off + a, b_top - (d - a), 0, c, r);
 }
 
+// Co-rank on the bitonic layout with block splitters: runs whose lengths and
+// offsets are multiples of IPT, a diagonal d that is one too, and for every
+// IPT-item block of a run (block k of the run at LDS offset o: splitter index
+// o / IPT + k, in the run's ascending order whatever its storage direction)
+// its first item in spF and its last in spL.  The lifting steps of IPT items
+// and more compare splitters -- A[t-1] is the last item of A's block t/IPT - 1,
+// B[d-t] the first of B's block (d-t)/IPT -- on small unswizzled arrays, six
+// instructions per step instead of ten; the last log2(IPT) steps run on the
+// items themselves.  The same answer as lds_corank_lift_rev for any top with
+// 2 * top > min(la, lb).
+template <int IPT, typename K>
+__device__ __forceinline__ int lds_corank_split_rev(const K *sk, const K *spF, const K *spL, int a_off, int la,
+                                                    int b_off, int lb, int b_top, int diag, int top) {
+    static_assert((IPT & (IPT - 1)) == 0 && IPT >= (int)sizeof(K), "IPT: a power of two, at least the key bytes");
+    constexpr uint32_t W = (uint32_t)sizeof(K), SH = (uint32_t)__builtin_ctz((unsigned)IPT / W);
+    // every length and offset here is a non-negative multiple of IPT: an
+    // item count x is x >> SH splitter bytes
+    const uint32_t lo = (uint32_t)(diag > lb ? diag - lb : 0);
+    const uint32_t hi = (uint32_t)(diag < la ? diag : la);
+    // block-granular steps: A[t-1] at pA + (t >> SH), B[diag-t] at pB - (t >> SH)
+    const char *pA = (const char *)spL + ((uint32_t)a_off >> SH) - W;
+    const char *pB = (const char *)spF + ((uint32_t)(b_off + diag) >> SH);
+    const uint32_t hiq = hi >> SH;
+    uint32_t abq = lo >> SH;
+    for (uint32_t s = (uint32_t)top >> SH; s >= W; s >>= 1) {
+        const uint32_t tq = min(abq + s, hiq);
+        const K ka = *(const K *)(pA + tq);
+        const K kb = *(const K *)(pB - tq);
+        abq = kb < ka ? abq : tq;
+    }
+    // the last steps on the items (in bytes), from the block-granular answer
+    const int pa = (a_off - 1) * (int)W, pb = (b_top - diag) * (int)W, hib = (int)(hi * W);
+    int ab = (int)(abq * (uint32_t)IPT);
+#pragma unroll
+    for (int s = IPT / 2 * (int)W; s >= (int)W; s >>= 1) {
+        const int tb = min(ab + s, hib);
+        const K ka = lds_at_bytes(sk, pa + tb);
+        const K kb = lds_at_bytes(sk, pb + tb);
+        ab = kb < ka ? ab : tb;
+    }
+    return (int)((uint32_t)ab / W);
+}
+
 // Largest power of two <= x (0 for 0): the co-rank step bound for runs whose
 // shorter side has x items.
 __device__ __forceinline__ int pow2_floor(int x) { return x > 0 ? 1 << (31 - __clz(x)) : 0; }
@@ -527,11 +570,34 @@ __device__ __forceinline__ void lds_put(K *sk, uint64_t *sv, int base, const K (
 
 // Stores IPT registers, items [o, o + IPT) of a run that the next merge reads
 // descending, at LDS indices top, top - 1, ..., top - IPT + 1 (top + 1 a
-// multiple of the chunk size) as 16-byte stores.
+// multiple of IPT, which every caller's block alignment gives) as 16-byte
+// stores.
 template <int IPT, typename K>
 __device__ __forceinline__ void lds_put_desc(K *sk, int top, const K (&r)[IPT]) {
     constexpr int E = chunk_items<K>();
     static_assert(IPT % E == 0, "IPT must fill whole 16-byte chunks");
+    if constexpr (IPT * sizeof(K) <= 128) {
+        // top + 1 is a multiple of IPT here, so the IPT items are one aligned
+        // block inside a 128-byte row: every chunk shares the row's swizzle
+        // term, and chunk k of the block (from its low end) is at
+        // (base ^ term) ^ 16 k
+        const int base = (top + 1 - IPT) * (int)sizeof(K);
+        const int sw = base ^ ((base >> 3) & 0x70);
+#pragma unroll
+        for (int c = 0; c < IPT; c += E) {
+            const int addr = sw ^ ((IPT - E - c) * (int)sizeof(K));
+            if constexpr (E == 4) {
+                uint4 q = {(uint32_t)r[c + 3], (uint32_t)r[c + 2], (uint32_t)r[c + 1], (uint32_t)r[c]};
+                *(uint4 *)((char *)sk + addr) = q;
+            } else {
+                ulonglong2 q;
+                q.x = (unsigned long long)r[c + 1];
+                q.y = (unsigned long long)r[c];
+                *(ulonglong2 *)((char *)sk + addr) = q;
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int c = 0; c < IPT; c += E) {
         const int lo = top - c - (E - 1);

@@ -141,6 +141,31 @@ def merge_rev(L, ia, ib, ipt):
 KMAX = 2**32 - 1
 
 
+def corank_split_rev(L, spF, spL, ipt, a_off, la, b_off, lb, b_top, diag, top):
+    """lds_corank_split_rev: lifting steps of ipt items and more on the block
+    splitters (A[t-1] = last item of A's block t/ipt - 1 = spL[a_off/ipt + t/ipt - 1],
+    B[diag-t] = first item of B's block (diag-t)/ipt = spF[(b_off + diag - t)/ipt]),
+    then the last log2(ipt) steps on the items."""
+    lo = max(0, diag - lb)
+    hi = min(diag, la)
+    ab = lo
+    s = top
+    while s >= ipt:
+        t = min(ab + s, hi)
+        ka = spL[a_off // ipt + t // ipt - 1]
+        kb = spF[(b_off + diag - t) // ipt]
+        ab = ab if kb < ka else t
+        s >>= 1
+    s = ipt // 2
+    while s > 0:
+        t = min(ab + s, hi)
+        ka = L[a_off + t - 1]
+        kb = L[b_top - diag + t]
+        ab = ab if kb < ka else t
+        s >>= 1
+    return ab
+
+
 def model_tile(runs, lo, hi, block, ipt):
     """k_kway_merge on one tile: returns the tile's output (first cnt items)."""
     kw = len(runs)
@@ -181,6 +206,24 @@ def model_tile(runs, lo, hi, block, ipt):
             L[e] = int(runs[j0][base - r if desc else base + r])
         else:
             L[e] = KMAX
+    # the slices' block splitters, as the kernel writes them after the loads;
+    # entries no thread wrote (the guards at -1 and block, blocks past the tile's
+    # items) read as garbage, as LDS would
+    class Garbage(dict):
+        def __missing__(self, k):
+            return (k * 2654435761 + Np) % 2**32
+    spF, spL = Garbage(), Garbage()
+    for tid in range(block):
+        c0 = tid * ipt
+        if c0 >= Np:
+            continue
+        j0 = sum(1 for j in range(1, kw) if P[j] <= c0)
+        first, last = L[c0], L[c0 + ipt - 1]
+        if j0 & 1:
+            idx = (P[j0] + P[j0 + 1] - c0) // ipt - 1
+            spF[idx], spL[idx] = last, first
+        else:
+            spF[tid], spL[tid] = first, last
     logkw = kw.bit_length() - 1
     for lev in range(1, logkw + 1):
         regs = {}
@@ -196,7 +239,8 @@ def model_tile(runs, lo, hi, block, ipt):
             s0 = (j0 >> lev) << lev
             a_off, mid, end = P[s0], P[s0 + hv], P[s0 + 2 * hv]
             la, lb, diag = mid - a_off, end - mid, c0 - a_off
-            a = corank_rev(L, a_off, la, end - 1, lb, diag, top)
+            a = corank_split_rev(L, spF, spL, ipt, a_off, la, mid, lb, end - 1, diag, top)
+            assert a == corank_rev(L, a_off, la, end - 1, lb, diag, top), (lev, tid)
             regs[tid] = (merge_rev(L, a_off + a, end - 1 - (diag - a), ipt), j0, end, diag)
         for tid, (r, j0, end, diag) in regs.items():
             c0 = tid * ipt
@@ -207,6 +251,8 @@ def model_tile(runs, lo, hi, block, ipt):
             else:
                 for i in range(ipt):
                     L[c0 + i] = r[i]
+            if lev < logkw:
+                spF[tid], spL[tid] = r[0], r[ipt - 1]
     cnt = sum(n)
     return L[:cnt]
 
