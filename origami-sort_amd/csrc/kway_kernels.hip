@@ -122,21 +122,20 @@ __device__ __forceinline__ uint64_t group_search(uint64_t lo, uint64_t hi, int l
 // tile, at least 4 per run), else one.
 template <int KW> constexpr int kw_bounds_tpw() { return KW <= 8 ? 2 : 1; }
 
+// The splits of tile t of the pass over the n items at `in` (t < that pass's
+// tiles when `valid`); every lane of the wave calls it, the lanes of half h
+// (when two tiles share a wave) for their own tile.
 template <typename K, bool HV, int KW>
-__global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2, int slog2,
-                                                     uint64_t Q, uint64_t ntg, uint64_t ntiles,
-                                                     const void *__restrict__ s0, const void *__restrict__ sm,
-                                                     uint64_t *__restrict__ tab) {
+__device__ __forceinline__ void kway_bounds_tile(const void *__restrict__ in, uint64_t n, int rlog2, int slog2,
+                                                 uint64_t Q, uint64_t ntg, uint64_t t, bool valid,
+                                                 const void *__restrict__ s0, const void *__restrict__ sm,
+                                                 uint64_t *__restrict__ tab) {
     constexpr int TPW = kw_bounds_tpw<KW>();    // tiles per wave
     constexpr int LPT = 64 / TPW;               // lanes per tile: [h * LPT, (h + 1) * LPT) take tile h
     constexpr int NW = KW * 8 / LPT > 1 ? KW * 8 / LPT : 1;  // window samples per lane
     constexpr int G = LPT / KW;                 // lanes per run
     const int lane = threadIdx.x & 63;
     const int h = lane / LPT, hl = lane % LPT;
-    const uint64_t t0 = ((uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * TPW;
-    if (t0 >= ntiles) return;  // uniform in the wave
-    const uint64_t t = t0 + (uint64_t)h;
-    const bool valid = t < ntiles;
     const uint64_t R = 1ull << rlog2;
     const uint64_t g = t / ntg, tl = t % ntg;
     const uint64_t gbase = g * KW * R;
@@ -227,6 +226,19 @@ __global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in
     tab[t * KW + j] = split;
 }
 
+template <typename K, bool HV, int KW>
+__global__ __launch_bounds__(256) void k_kway_bounds(const void *__restrict__ in, uint64_t n, int rlog2, int slog2,
+                                                     uint64_t Q, uint64_t ntg, uint64_t ntiles,
+                                                     const void *__restrict__ s0, const void *__restrict__ sm,
+                                                     uint64_t *__restrict__ tab) {
+    constexpr int TPW = kw_bounds_tpw<KW>();
+    const int h = (int)(threadIdx.x & 63) / (64 / TPW);
+    const uint64_t t0 = ((uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * TPW;
+    if (t0 >= ntiles) return;  // uniform in the wave
+    const uint64_t t = t0 + (uint64_t)h;
+    kway_bounds_tile<K, HV, KW>(in, n, rlog2, slog2, Q, ntg, t, t < ntiles, s0, sm, tab);
+}
+
 // Inclusive sums across lanes 0 .. KW-1 of a wave (KW <= 32): DPP row shifts
 // give the sums within each row of 16 lanes (lanes with no source add 0); the
 // lanes of the second row then add the first row's total, lane 15's sum.
@@ -274,11 +286,13 @@ __device__ __forceinline__ void store_items(const K *sp0, K *ob, int tid) {
 // ----------------------------------------------------------------------------
 // The merge: one workgroup per tile.
 // ----------------------------------------------------------------------------
+// Merges tile t of the pass over the n items at `in` into `out` (one
+// workgroup).
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
-__global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
-                                                      int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
-                                                      const uint64_t *__restrict__ tab, SampleSpec sp,
-                                                      uint32_t *__restrict__ err) {
+__device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
+                                                int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
+                                                const uint64_t *__restrict__ tab, const SampleSpec &sp,
+                                                uint32_t *__restrict__ err, const uint64_t t) {
     static_assert(KW == 2 || KW == 4 || KW == 8 || KW == 16 || KW == 32, "fan-in 2, 4, 8, 16 or 32");
     static_assert((IPT & (IPT - 1)) == 0 && IPT >= 4, "IPT: a power of two >= 4");
     constexpr int C = BLOCK * IPT;
@@ -303,7 +317,6 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
     K *sk = skb + lds_front<K>();
     uint64_t *sv = svb + (HV ? lds_front<K>() : 0);
     const int tid = threadIdx.x, lane = tid & 63;
-    const uint64_t t = blockIdx.x;
     const uint64_t R = 1ull << rlog2;
     const uint64_t g = t / ntg, tl = t % ntg;
     const uint64_t gbase = g * KW * R;
@@ -596,6 +609,14 @@ __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ i
 }
 
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
+                                                      int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
+                                                      const uint64_t *__restrict__ tab, SampleSpec sp,
+                                                      uint32_t *__restrict__ err) {
+    kway_merge_tile<K, HV, KW, BLOCK, IPT>(in, out, n, rlog2, ntg, nt_last, cout, tab, sp, err, blockIdx.x);
+}
+
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
 static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, int slog2, int cout, uint64_t *tab,
                               const void *s0, const void *sm, const SampleSpec *next, hipStream_t s, hipEvent_t mid,
                               uint32_t *err) {
@@ -617,12 +638,82 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     return hipGetLastError();
 }
 
+// ----------------------------------------------------------------------------
+// Batched form: one pass of many segments' K-way plans (segments of a batched
+// sort), all of fan-in KW at this pass.  Runs are 2^rlog2 items in every
+// listed segment (each pass before a segment's last is of the full fan-in), so
+// the tile geometry -- Q, tiles per full group -- is the same for all; a tile
+// map gives each global tile its segment and its index there.
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_kway_tilemap(const uint32_t *__restrict__ list,
+                                                      const uint32_t *__restrict__ tfirst, uint2 *__restrict__ tmap) {
+    const uint32_t i = blockIdx.x;
+    const uint32_t a = tfirst[i], b = tfirst[i + 1], seg = list[i];
+    for (uint32_t k = threadIdx.x; a + k < b; k += blockDim.x) tmap[a + k] = make_uint2(seg, k);
+}
+
+hipError_t launch_kway_tilemap(const uint32_t *list, const uint32_t *tfirst, uint32_t nlist, uint2 *tmap,
+                               hipStream_t s) {
+    if (nlist == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_kway_tilemap, dim3(nlist), dim3(256), 0, s, list, tfirst, tmap);
+    return hipGetLastError();
+}
+
+template <typename K, bool HV, int KW>
+__global__ __launch_bounds__(256) void k_seg_kway_bounds(const KSegH *__restrict__ segs, const uint2 *__restrict__ tmap,
+                                                         uint64_t ntiles, int pass, int rlog2, int slog2, uint64_t Q,
+                                                         uint64_t ntg, int smi) {
+    constexpr int TPW = kw_bounds_tpw<KW>();
+    const int h = (int)(threadIdx.x & 63) / (64 / TPW);
+    const uint64_t t0 = ((uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6)) * TPW;
+    if (t0 >= ntiles) return;  // uniform in the wave
+    const uint64_t tg = t0 + (uint64_t)h;
+    const bool valid = tg < ntiles;
+    const uint2 m = tmap[valid ? tg : ntiles - 1];
+    const KSegH &g = segs[m.x];
+    kway_bounds_tile<K, HV, KW>(g.buf[pass & 1], g.len, rlog2, slog2, Q, ntg, m.y, valid, g.smp[0], g.smp[smi],
+                                g.tab);
+}
+
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_seg_kway_merge(const KSegH *__restrict__ segs,
+                                                          const uint2 *__restrict__ tmap, int pass, int rlog2,
+                                                          int slog2, uint64_t Q, uint64_t ntg,
+                                                          uint32_t *__restrict__ err) {
+    const uint2 m = tmap[blockIdx.x];
+    const KSegH &g = segs[m.x];
+    const uint64_t nt_last = kw_group_tiles(kw_last_samples(g.len, rlog2, slog2, KW), Q);
+    const bool last = pass + 1 >= (int)g.passes;
+    SampleSpec sp = {nullptr, 0, 0, 0};
+    if (!last) sp = {g.smp[0], rlog2 + (int)g.lv[pass], 1 << g.lv[pass + 1], (int)g.sl[pass + 1]};
+    kway_merge_tile<K, HV, KW, BLOCK, IPT>(g.buf[pass & 1], g.buf[(pass + 1) & 1], g.len, rlog2, ntg, nt_last,
+                                           last ? (int)g.cout : 0, g.tab, sp, err, m.y);
+}
+
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
+static hipError_t launch_seg_kway(const KSegH *segs, const uint2 *tmap, uint64_t ntiles, int pass, int rlog2,
+                                  int slog2, int smi, uint32_t *err, hipStream_t s) {
+    constexpr uint64_t C = (uint64_t)BLOCK * IPT;
+    if (ntiles == 0) return hipSuccess;
+    const uint64_t Q = kw_q(C, IPT, KW, slog2);
+    if (Q == 0 || rlog2 - slog2 > 27 || rlog2 < slog2) return hipErrorInvalidValue;
+    const uint64_t ntg = kw_group_tiles(((1ull << rlog2) * KW) >> slog2, Q);
+    const uint64_t tpb = 4 * (uint64_t)kw_bounds_tpw<KW>();  // tiles per 256-thread block
+    hipLaunchKernelGGL((k_seg_kway_bounds<K, HV, KW>), dim3((unsigned)((ntiles + tpb - 1) / tpb)), dim3(256), 0, s,
+                       segs, tmap, ntiles, pass, rlog2, slog2, Q, ntg, smi);
+    hipLaunchKernelGGL((k_seg_kway_merge<K, HV, KW, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, segs,
+                       tmap, pass, rlog2, slog2, Q, ntg, err);
+    return hipGetLastError();
+}
+
 #define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16) X(1024, 32) X(1024, 16)
 #define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8) X(1024, 16) X(1024, 8)
 #define OGS_KWAY_SHAPES_KV(X) X(512, 8) X(256, 16) X(256, 8) X(512, 4) X(1024, 4)
 #define OGS_KX(K, HV, B, I)                                                                                 \
     {B, I, {&launch_kway<K, HV, 2, B, I>, &launch_kway<K, HV, 4, B, I>, &launch_kway<K, HV, 8, B, I>,       \
-            &launch_kway<K, HV, 16, B, I>, &launch_kway<K, HV, 32, B, I>}},
+            &launch_kway<K, HV, 16, B, I>, &launch_kway<K, HV, 32, B, I>},                                  \
+     {&launch_seg_kway<K, HV, 2, B, I>, &launch_seg_kway<K, HV, 4, B, I>, &launch_seg_kway<K, HV, 8, B, I>, \
+      &launch_seg_kway<K, HV, 16, B, I>, &launch_seg_kway<K, HV, 32, B, I>}},
 
 static const KwayShapeFns kKway32[] = {
 #define X(B, I) OGS_KX(uint32_t, false, B, I)

@@ -558,6 +558,26 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
     b.lanes = tune[ki.kc].v[TP_PARTITION_LANES];
     (void)w;
     b.MT = MT;
+    // The K-way plan for the large class: every pass of fan-in 2^lmax but a
+    // segment's last, so every segment has runs of 2^(rl0 + lmax p) at pass p.
+    // Samples of 32-bit keys are u64 records (key, run, index), merged as u64
+    // keys; those of 64-bit keys {key, run << 32 | index} records, merged by
+    // the stable key-value merge.
+    const int lmax = tune[ki.kc].v[TP_KWAY_FANIN] >= 2 ? ilog2((uint64_t)tune[ki.kc].v[TP_KWAY_FANIN]) : 0;
+    b.rl0 = (LT & (LT - 1)) == 0 ? ilog2(LT) : 0;
+    b.kway = lmax > 0 && b.rl0 >= 11;
+    b.skc = ki.kc == KC_32 ? KC_64 : KC_KV;
+    b.sw = ki.kc == KC_32 ? 8 : 16;
+    if (b.kway) {
+        int kcnt = 0;
+        const KwayShapeFns *kt = kway_shapes(ki.kc, &kcnt);
+        int kidx = tune[ki.kc].v[TP_KWAY_SHAPE];
+        if (kidx < 0 || kidx >= kcnt) kidx = 0;
+        b.ks = &kt[kidx];
+        const int sl0 = std::min(tune[ki.kc].v[TP_KWAY_SAMPLE], b.rl0 - 8);
+        b.kway = kway_tiles(1, b.rl0, sl0, 1 << lmax, b.ks->tile(), b.ks->ipt) > 0;
+        b.lmax = lmax;
+    }
     std::vector<size_t> large;  // indices into b.recs
     for (size_t i = 0; i < nseg; i++) {
         const SegSpec &g = segs[i];
@@ -579,12 +599,34 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
             int p = 0;
             for (uint64_t R = LT; R < g.len; R *= 2) p++;
             r.cls = 2;
-            r.passes = (uint32_t)p;
             r.tidx = b.ntd[2];
-            r.lidx = (uint32_t)large.size();
             b.ntd[2] += (g.len + LT - 1) / LT;
-            b.max_p = std::max(b.max_p, p);
-            large.push_back(b.recs.size());
+            if (b.kway && p > 0) {
+                BatchBuild::KSeg k{};
+                k.seg = b.recs.size();
+                k.passes = (p + lmax - 1) / lmax;
+                int rl = b.rl0, smin = 63;
+                for (int q = 0; q < k.passes; q++) {
+                    k.lv[q] = (uint8_t)(q + 1 < k.passes ? lmax : p - lmax * (k.passes - 1));
+                    k.sl[q] = (uint8_t)std::min(tune[ki.kc].v[TP_KWAY_SAMPLE], rl - 8);
+                    if (rl - k.sl[q] > 27)
+                        return fail(OGS_ERR_ARG, "segment %zu: no sample spacing for runs of 2^%d", i, rl);
+                    smin = std::min(smin, (int)k.sl[q]);
+                    const uint64_t nt = kway_tiles(g.len, rl, k.sl[q], 1 << k.lv[q], b.ks->tile(), b.ks->ipt);
+                    k.tab = std::max<uint64_t>(k.tab, nt << k.lv[q]);
+                    rl += k.lv[q];
+                }
+                k.ns = (g.len + (1ull << smin) - 1) >> smin;  // records of sw bytes
+                r.passes = (uint32_t)k.passes;
+                r.kseg1 = (uint32_t)b.kseg.size() + 1;
+                b.kpasses = std::max(b.kpasses, k.passes);
+                b.kseg.push_back(k);
+            } else {
+                r.passes = (uint32_t)p;
+                r.lidx = (uint32_t)large.size();
+                b.max_p = std::max(b.max_p, p);
+                large.push_back(b.recs.size());
+            }
         }
         b.recs.push_back(r);
     }
@@ -616,13 +658,171 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
     b.bytes = align(b.bytes + b.recs.size() * sizeof(SegRecH));
     b.o_poff = b.bytes;
     b.bytes = align(b.bytes + b.poff.size() * sizeof(uint64_t));
-    if (b.recs.empty()) b.bytes = 0;
+    if (!b.kseg.empty()) {
+        b.o_kseg = b.bytes;
+        b.bytes = align(b.bytes + b.kseg.size() * sizeof(KSegH));
+        // per pass: the sample merges, then one split + merge launch per fan-in
+        const uint64_t C = b.ks->tile();
+        int rl = b.rl0;
+        for (int p = 0; p < b.kpasses; p++) {
+            int lvmax = 0, sl = 0;
+            for (const auto &k : b.kseg)
+                if (k.passes > p) {
+                    lvmax = std::max(lvmax, (int)k.lv[p]);
+                    sl = k.sl[p];  // the same for every segment: it depends on rl alone
+                }
+            // group samples sorted by one u64 tile sort when a tile holds a group
+            // (the first pass: groups of 2^lv runs of 2^(rl - sl) samples), else
+            // lv levels of 2-way merges
+            const uint64_t span = 1ull << (rl - sl + lvmax);
+            const BlockShapeFns *sbs = nullptr;
+            if (p == 0 && ki.kc == KC_32) {  // u64 samples are unique: a sort is the merge
+                int bcnt = 0;
+                const BlockShapeFns *bt = block_shapes(KC_64, &bcnt);
+                for (int j = 0; j < bcnt; j++)
+                    if (bt[j].tile() >= span && (!sbs || bt[j].tile() < sbs->tile())) sbs = &bt[j];
+            }
+            if (sbs) {
+                BatchBuild::SMerge m;
+                m.pass = p;
+                m.level = -1;
+                m.bs = sbs;
+                for (const auto &k : b.kseg) {
+                    if (k.passes <= p) continue;
+                    const uint64_t g = (uint64_t)1 << (rl - sl + k.lv[p]);
+                    const uint64_t nsp = (b.recs[k.seg].len + (1ull << sl) - 1) >> sl;
+                    m.ndesc += (nsp + g - 1) / g;
+                }
+                m.o_desc = b.bytes;
+                b.bytes = align(b.bytes + m.ndesc * sizeof(TileDescH));
+                b.smerge.push_back(m);
+            } else {
+                int mcnt = 0;
+                const MergeShapeFns *mt = merge_shapes(b.skc, &mcnt);
+                for (int q = 0; q < lvmax; q++) {
+                    BatchBuild::SMerge m;
+                    m.pass = p;
+                    m.level = q;
+                    m.R = 1ull << (rl - sl + q);
+                    for (int j = 0; j < mcnt; j++)
+                        if (mt[j].tile() <= 2 * m.R && (!m.ms || mt[j].tile() > m.ms->tile())) m.ms = &mt[j];
+                    if (!m.ms) return fail(OGS_ERR_ARG, "no sample merge tile divides 2^%d", rl - sl + q + 1);
+                    for (const auto &k : b.kseg) {
+                        if (k.passes <= p || k.lv[p] <= q) continue;
+                        const uint64_t nsp = (b.recs[k.seg].len + (1ull << sl) - 1) >> sl;
+                        m.ndesc += (nsp + m.ms->tile() - 1) / m.ms->tile();
+                    }
+                    m.o_desc = b.bytes;
+                    b.bytes = align(b.bytes + m.ndesc * sizeof(MergeDescH));
+                    b.smerge.push_back(m);
+                }
+            }
+            for (int lvv = 1; lvv <= lvmax; lvv++) {
+                BatchBuild::KLaunch L;
+                L.pass = p;
+                L.lv = lvv;
+                L.smi = sbs ? 1 : 1 + ((lvv - 1) & 1);
+                L.tfirst.push_back(0);
+                for (size_t i = 0; i < b.kseg.size(); i++) {
+                    const auto &k = b.kseg[i];
+                    if (k.passes <= p || k.lv[p] != lvv) continue;
+                    const uint64_t nt = kway_tiles(b.recs[k.seg].len, rl, sl, 1 << lvv, C, b.ks->ipt);
+                    if (L.tfirst.back() + nt >= (1ull << 32)) return fail(OGS_ERR_ARG, "batch too large for one K-way launch");
+                    L.list.push_back((uint32_t)i);
+                    L.tfirst.push_back(L.tfirst.back() + (uint32_t)nt);
+                }
+                if (L.list.empty()) continue;
+                L.o_list = b.bytes;
+                b.bytes = align(b.bytes + L.list.size() * sizeof(uint32_t));
+                L.o_tfirst = b.bytes;
+                b.bytes = align(b.bytes + L.tfirst.size() * sizeof(uint32_t));
+                b.klaunch.push_back(std::move(L));
+            }
+            rl += lmax;  // every pass before a segment's last is of the full fan-in
+        }
+    }
+    b.up_end = b.bytes;
+    // device-only regions: tile maps, then each segment's samples and split table
+    for (auto &L : b.klaunch) {
+        L.o_tmap = b.bytes;
+        b.bytes = align(b.bytes + (size_t)L.tfirst.back() * sizeof(uint2));
+    }
+    for (auto &k : b.kseg) {
+        k.o_data = b.bytes;
+        b.bytes = align(b.bytes + 3 * align(k.ns * b.sw) + k.tab * sizeof(uint64_t));
+    }
+    if (b.recs.empty()) b.bytes = b.up_end = 0;
     return OGS_OK;
 }
 
-void BatchBuild::pack(uint8_t *h) const {
+void BatchBuild::pack(uint8_t *h, uint8_t *D, const SegSpec *segs) const {
+    (void)segs;
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     if (!recs.empty()) memcpy(h, recs.data(), recs.size() * sizeof(SegRecH));
     if (!poff.empty()) memcpy(h + (o_poff - o_seg), poff.data(), poff.size() * sizeof(uint64_t));
+    if (kseg.empty()) return;
+    // the K-way records, with device pointers into the table at D
+    std::vector<KSegH> ks(kseg.size());
+    for (size_t i = 0; i < kseg.size(); i++) {
+        const KSeg &k = kseg[i];
+        const SegRecH &r = recs[k.seg];
+        KSegH &x = ks[i];
+        memset(&x, 0, sizeof(x));
+        x.buf[0] = (k.passes & 1) ? r.other : r.fin;
+        x.buf[1] = (k.passes & 1) ? r.fin : r.other;
+        x.len = r.len;
+        const size_t sb = align(k.ns * sw);
+        for (int j = 0; j < 3; j++) x.smp[j] = D + k.o_data + j * sb;
+        x.tab = (uint64_t *)(D + k.o_data + 3 * sb);
+        x.passes = (uint32_t)k.passes;
+        x.cout = (uint32_t)ki.codec;
+        memcpy(x.lv, k.lv, 8);
+        memcpy(x.sl, k.sl, 8);
+    }
+    memcpy(h + (o_kseg - o_seg), ks.data(), ks.size() * sizeof(KSegH));
+    for (const KLaunch &L : klaunch) {
+        memcpy(h + (L.o_list - o_seg), L.list.data(), L.list.size() * sizeof(uint32_t));
+        memcpy(h + (L.o_tfirst - o_seg), L.tfirst.data(), L.tfirst.size() * sizeof(uint32_t));
+    }
+    // the sample merges' descriptors
+    int rl = rl0;
+    for (int p = 0; p < kpasses; p++) {
+        for (const SMerge &m : smerge) {
+            if (m.pass != p) continue;
+            uint8_t *dst = h + (m.o_desc - o_seg);
+            size_t o = 0;
+            for (size_t i = 0; i < kseg.size(); i++) {
+                const KSeg &k = kseg[i];
+                if (k.passes <= p) continue;
+                const int sl = k.sl[p];
+                const uint64_t nsp = (recs[k.seg].len + (1ull << sl) - 1) >> sl;
+                char *smp = (char *)ks[i].smp[0];
+                const size_t sb = align(k.ns * sw);
+                if (m.level < 0) {
+                    // the tile sort of each group's samples, smp[0] -> smp[1]
+                    const uint64_t g = (uint64_t)1 << (rl - sl + k.lv[p]);
+                    for (uint64_t g0 = 0; g0 < nsp; g0 += g) {
+                        const uint64_t c = nsp - g0 < g ? nsp - g0 : g;
+                        TileDescH t{smp + g0 * sw, smp + sb + g0 * sw, (uint32_t)c, 0};
+                        memcpy(dst + o * sizeof(TileDescH), &t, sizeof(t));
+                        o++;
+                    }
+                } else {
+                    if (k.lv[p] <= m.level) continue;
+                    const int q = m.level;
+                    const char *src = smp + (q == 0 ? 0 : (1 + ((q - 1) & 1)) * sb);
+                    char *dd = smp + (1 + (q & 1)) * sb;
+                    const uint64_t MT = m.ms->tile();
+                    for (uint64_t o0 = 0; o0 < nsp; o0 += MT) {
+                        MergeDescH md{src, dd, nsp, o0};
+                        memcpy(dst + o * sizeof(MergeDescH), &md, sizeof(md));
+                        o++;
+                    }
+                }
+            }
+        }
+        rl += lmax;  // every pass before a segment's last is of the full fan-in
+    }
 }
 
 // Writes the batch's descriptors into the device table at D (whose records
@@ -642,15 +842,40 @@ int Ctx::batch_expand(const BatchBuild &b, void *D, hipStream_t s) {
 // kernel launches only, so it can be captured into a graph.
 int Ctx::batch_enqueue(const BatchBuild &b, void *D, uint64_t *part, hipStream_t s) {
     uint8_t *d = (uint8_t *)D;
+    const KSegH *kseg = b.kseg.empty() ? nullptr : (const KSegH *)(d + b.o_kseg);
     for (int c = 0; c < 3; c++) {
         if (!b.ntd[c]) continue;
-        hipError_t e = b.cls[c]->run_seg(d + b.o_td[c], b.ntd[c], b.ki.codec, b.ki.codec, s);
+        hipError_t e = b.cls[c]->run_seg(d + b.o_td[c], b.ntd[c], b.ki.codec, b.ki.codec, c == 2 ? kseg : nullptr, s);
         if (e != hipSuccess) return hip_fail(e, "batched tile-sort launch");
     }
     for (int p = 0; p < b.max_p; p++) {
         hipError_t e = b.mb->run_seg(d + b.o_desc + b.pass_start[p] * sizeof(MergeDescH), b.pass_cnt[p], b.LT << p,
                                      b.ki.codec, b.seg_table ? part : nullptr, b.lanes, s);
         if (e != hipSuccess) return hip_fail(e, "batched merge launch");
+    }
+    // K-way passes of the large class: per pass the group samples merged, then
+    // per fan-in the tile map, the splits and the merge
+    int rl = b.rl0;
+    for (int p = 0; p < b.kpasses; p++) {
+        for (const auto &m : b.smerge) {
+            if (m.pass != p || !m.ndesc) continue;
+            hipError_t e = m.level < 0 ? m.bs->run_seg(d + m.o_desc, m.ndesc, 0, 0, nullptr, s)
+                                       : m.ms->run_seg(d + m.o_desc, m.ndesc, m.R, 0, nullptr, b.lanes, s);
+            if (e != hipSuccess) return hip_fail(e, "batched sample merge launch");
+        }
+        for (const auto &L : b.klaunch) {
+            if (L.pass != p) continue;
+            int sl = 0;
+            for (const auto &k : b.kseg)
+                if (k.passes > p) sl = k.sl[p];
+            hipError_t e = launch_kway_tilemap((const uint32_t *)(d + L.o_list), (const uint32_t *)(d + L.o_tfirst),
+                                               (uint32_t)L.list.size(), (uint2 *)(d + L.o_tmap), s);
+            if (e == hipSuccess)
+                e = b.ks->run_seg[L.lv - 1](kseg, (const uint2 *)(d + L.o_tmap), L.tfirst.back(), p, rl, sl, L.smi,
+                                            err, s);
+            if (e != hipSuccess) return hip_fail(e, "batched K-way pass launch");
+        }
+        rl += b.lmax;
     }
     return OGS_OK;
 }
@@ -682,9 +907,9 @@ int Ctx::batch(ogs_key_t kt, const SegSpec *segs, size_t nseg, hipStream_t s) {
     e = hipEventSynchronize(ws_->staged);
     if (e != hipSuccess) return hip_fail(e, "staging wait");
     std::vector<uint8_t> &h = ws_->host;
-    h.resize(b.bytes - b.o_seg);
-    b.pack(h.data());
-    e = hipMemcpyAsync((uint8_t *)ws_->dev + b.o_seg, h.data(), b.bytes - b.o_seg, hipMemcpyHostToDevice, s);
+    h.resize(b.up_end - b.o_seg);
+    b.pack(h.data(), (uint8_t *)ws_->dev, segs);
+    e = hipMemcpyAsync((uint8_t *)ws_->dev + b.o_seg, h.data(), b.up_end - b.o_seg, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventRecord(ws_->staged, s);
     if (e != hipSuccess) return hip_fail(e, "segment table upload");
     rc = batch_expand(b, ws_->dev, s);
@@ -950,11 +1175,11 @@ int ogs_segmented_plan(ogs_ctx *ctx, ogs_key_t kt, void *d_data, void *d_scratch
         return hip_fail(err, what);
     };
     if (b.bytes) {
-        std::vector<uint8_t> h(b.bytes - b.o_seg);
-        b.pack(h.data());
         e = hipMalloc(&p->dev, b.bytes);
         if (e != hipSuccess) return bail(e, "hipMalloc(plan descriptors)");
-        e = hipMemcpy((uint8_t *)p->dev + b.o_seg, h.data(), b.bytes - b.o_seg, hipMemcpyHostToDevice);
+        std::vector<uint8_t> h(b.up_end - b.o_seg);
+        b.pack(h.data(), (uint8_t *)p->dev, segs.data());
+        e = hipMemcpy((uint8_t *)p->dev + b.o_seg, h.data(), b.up_end - b.o_seg, hipMemcpyHostToDevice);
         if (e != hipSuccess) return bail(e, "plan segment table upload");
         if (b.seg_table && b.max_pass) {
             e = hipMalloc(&p->part, b.max_pass * sizeof(uint64_t));

@@ -124,7 +124,43 @@ struct BatchBuild {
     bool seg_table = true;
     int lanes = 4;
     std::vector<size_t> plain;
-    void pack(uint8_t *h) const;  // bytes [o_seg, bytes) of the table, into h[0 ..)
+    // Large segments on the sample-split K-way plan (32-bit keys): their
+    // records, and per pass the launches by fan-in, the sample merges and the
+    // device regions (tile maps, sample arrays, split tables).
+    bool kway = false;
+    const KwayShapeFns *ks = nullptr;
+    struct KSeg {
+        size_t seg;              // index into recs
+        int passes;
+        uint8_t lv[8], sl[8];
+        uint64_t ns;             // sample records at the finest spacing
+        uint64_t tab;            // split-table entries
+        size_t o_data;           // its sample arrays and split table (device-only region)
+    };
+    std::vector<KSeg> kseg;
+    struct KLaunch {             // pass p, fan-in 2^lv: its segments (indices into kseg)
+        int pass, lv, smi;
+        std::vector<uint32_t> list, tfirst;
+        size_t o_list = 0, o_tfirst = 0, o_tmap = 0;
+    };
+    std::vector<KLaunch> klaunch;
+    struct SMerge {              // one launch of the sample merges: a tile sort or a 2-way level
+        int pass, level;         // level -1: the tile sort of each group's samples
+        uint64_t R = 0;          // 2-way: run length in samples
+        const BlockShapeFns *bs = nullptr;
+        const MergeShapeFns *ms = nullptr;
+        uint64_t ndesc = 0;
+        size_t o_desc = 0;
+    };
+    std::vector<SMerge> smerge;
+    int kpasses = 0;             // the most K-way passes of any segment
+    int lmax = 0;                // log2 of the full fan-in
+    KeyClass skc = KC_64;        // key class the samples are merged as
+    uint64_t sw = 8;             // bytes per sample record
+    int rl0 = 0;                 // log2 of the large tile
+    size_t o_kseg = 0, up_end = 0;  // uploaded bytes are [o_seg, up_end); the rest is device-only
+    // bytes [o_seg, up_end) of the table into h[0 ..), device pointers relative to D
+    void pack(uint8_t *h, uint8_t *D, const SegSpec *segs) const;
 };
 
 // Caller-provided device workspace for a plain sort (ogs_sort_noalloc): the

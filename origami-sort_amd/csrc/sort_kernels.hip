@@ -24,7 +24,8 @@ namespace ogs {
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
 __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t base, int cnt, int cin, int cout,
                                           K *sk, uint64_t *sv, int *smisc, const void *vin = nullptr,
-                                          void *vout = nullptr, const SampleSpec *sp = nullptr) {
+                                          void *vout = nullptr, const SampleSpec *sp = nullptr,
+                                          uint64_t sbase = ~0ull) {
     constexpr int TILE = BLOCK * IPT;
     constexpr K KMAX = KeyMax<K>::v;
     const int tid = threadIdx.x, lane = tid & 63;
@@ -130,7 +131,9 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     lds_store_tile<BLOCK, TILE, HV>(sk, sv, out, base, lim, cout, vout);
     // samples of the whole tile: in KV tiles the items past lim (maximum keys
     // written straight to the tail) sit in LDS as maximum keys too
-    if (sp && sp->smp) write_samples<BLOCK>(sk, cnt, base, sp->smp, sp->rlog2, sp->kw, sp->slog2);
+    // (the tile's first item is item sbase of the sampled array; base by default)
+    if (sp && sp->smp)
+        write_samples<BLOCK>(sk, cnt, sbase == ~0ull ? base : sbase, sp->smp, sp->rlog2, sp->kw, sp->slog2);
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
@@ -159,15 +162,28 @@ struct TileDesc {
 static_assert(sizeof(TileDesc) == 24, "descriptor layout is shared with the host");
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-__global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const TileDesc *__restrict__ tiles, int cin, int cout) {
+__global__ __launch_bounds__(BLOCK) void k_seg_tile_sort(const TileDesc *__restrict__ tiles, int cin, int cout,
+                                                         const KSegH *__restrict__ kseg) {
     constexpr int TILE = BLOCK * IPT;
     __shared__ __attribute__((aligned(16))) K skb[TILE + lds_extra<K, IPT>()];
     __shared__ __attribute__((aligned(16))) uint64_t sv[HV ? TILE + lds_back<K, IPT>() : 2];
     __shared__ int smisc[2];
     K *sk = skb + lds_front<K>();
     const TileDesc t = tiles[blockIdx.x];
-    const int cnt = (int)(t.cnt & 0x3FFFFFFFu);
-    tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, ((t.cnt >> 30) & 1u) ? cout : 0, sk, sv, smisc);
+    const int cnt = (int)(t.cnt & kTileCountMask);
+    const int co = ((t.cnt >> 30) & 1u) ? cout : 0;
+    if (t.cnt & kTileKwayBit) {
+        // a tile of a segment on the K-way plan: also the samples of its run
+        // for the segment's first K-way pass (the tile sits in the segment's
+        // buf[0] at item (out - buf[0]) / item size)
+        const KSegH &g = kseg[t.pad];
+        constexpr uint64_t W = HV ? 16 : sizeof(K);
+        const uint64_t g0 = (uint64_t)((const char *)t.out - (const char *)g.buf[0]) / W;
+        const SampleSpec sp = {g.smp[0], __builtin_ctz((unsigned)TILE), 1 << g.lv[0], (int)g.sl[0]};
+        tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, co, sk, sv, smisc, nullptr, nullptr, &sp, g0);
+        return;
+    }
+    tile_sort<K, HV, BLOCK, IPT, WLEV>(t.in, t.out, 0, cnt, cin, co, sk, sv, smisc);
 }
 
 // ----------------------------------------------------------------------------
@@ -397,10 +413,11 @@ static hipError_t launch_block(const void *in, void *out, uint64_t n, int cin, i
 }
 
 template <typename K, bool HV, int BLOCK, int IPT, int WLEV>
-static hipError_t launch_seg_block(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s) {
+static hipError_t launch_seg_block(const void *tiles, uint64_t ntiles, int cin, int cout, const KSegH *kseg,
+                                   hipStream_t s) {
     if (ntiles == 0) return hipSuccess;
     hipLaunchKernelGGL((k_seg_tile_sort<K, HV, BLOCK, IPT, WLEV>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s,
-                       (const TileDesc *)tiles, cin, cout);
+                       (const TileDesc *)tiles, cin, cout, kseg);
     return hipGetLastError();
 }
 
@@ -924,12 +941,15 @@ __global__ __launch_bounds__(64) void k_expand(const SegRecH *__restrict__ segs,
     const uint32_t p = g.passes;
     char *dst = (char *)((p & 1) ? g.other : g.fin);
     const uint32_t fin = p == 0 ? kTileFinalBit : 0u;
+    const uint32_t kw = g.kseg1 ? kTileKwayBit : 0u;
     const uint64_t nt = (g.len + LT - 1) / LT;
     for (uint64_t k = lane; k < nt; k += 64) {
         const uint64_t o = k * LT;
         const uint64_t c = g.len - o < LT ? g.len - o : LT;
-        td2[g.tidx + k] = TileDescH{(const char *)g.in + o * w, dst + o * w, (uint32_t)c | fin, 0};
+        td2[g.tidx + k] = TileDescH{(const char *)g.in + o * w, dst + o * w, (uint32_t)c | fin | kw,
+                                    g.kseg1 ? g.kseg1 - 1 : 0u};
     }
+    if (g.kseg1) return;  // the K-way passes take it from here
     const uint64_t nm = (g.len + MT - 1) / MT;
     for (uint32_t q = 0; q < p; q++) {
         // after q passes the data sits in fin iff (passes - q) is even

@@ -35,11 +35,37 @@ struct MergeDescH {
 static_assert(sizeof(TileDescH) == 24 && sizeof(MergeDescH) == 32, "descriptor layouts are shared with the kernels");
 static constexpr uint64_t kSegLastBit = 1ull << 62;
 static constexpr uint32_t kTileFinalBit = 1u << 30;
+// A large-class tile of a segment on the K-way plan: the tile sort also writes
+// its samples; `pad` is the segment's index in the batch's KSegH array.
+static constexpr uint32_t kTileKwayBit = 1u << 29;
+static constexpr uint32_t kTileCountMask = (1u << 29) - 1;
+
+// A large segment of a batched sort on the sample-split K-way plan (device
+// record).  Pass p reads buf[p & 1] and writes buf[(p + 1) & 1]; buf[0] is the
+// tile sort's output, so the last pass lands in the segment's result buffer.
+// smp[0] holds the tagged samples of the current runs (written by the kernel
+// that wrote the runs), smp[1] and smp[2] their group merges; tab is the split
+// table (tiles * fan-in entries of the segment's largest pass).  Passes run
+// 2^lv[p] runs of 2^(tile + sum of earlier lv) items into one, with samples
+// every 2^sl[p] items.
+struct KSegH {
+    void *buf[2];
+    uint64_t len;
+    uint64_t *tab;
+    void *smp[3];
+    uint32_t passes;
+    uint32_t cout;  // output codec of the last pass
+    uint8_t lv[8];
+    uint8_t sl[8];
+};
+static_assert(sizeof(KSegH) == 80, "record layout is shared with the kernels");
 
 // One segment of a batched sort as the descriptor-expansion kernel reads it:
 // its buffers and length, its size class (0 tiny, 1 small, 2 large), its
 // first tile descriptor in that class's table, and for the large class its
-// merge-pass count and its row in the per-pass offset table.
+// merge-pass count and either its row in the per-pass offset table of 2-way
+// merge descriptors or, on the K-way plan, its K-way record (kseg1 = index + 1;
+// 0 = 2-way plan).
 struct SegRecH {
     const void *in;
     void *fin;
@@ -49,7 +75,7 @@ struct SegRecH {
     uint32_t cls;
     uint32_t passes;
     uint32_t lidx;
-    uint32_t pad;
+    uint32_t kseg1;
 };
 // Writes every tile and merge descriptor of a batch on the device (one
 // wavefront per segment): td[c] are the class tables, md the merge table,
@@ -77,7 +103,10 @@ struct SampleSpec {
 // classes): samples of the sorted tiles for the first K-way pass.
 typedef hipError_t (*BlockLaunch)(const void *in, void *out, uint64_t n, int cin, int cout, hipStream_t s,
                                   const void *vin, void *vout, const SampleSpec *sp);
-typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin, int cout, hipStream_t s);
+// kseg: the batch's K-way segment records, for tiles flagged kTileKwayBit
+// (nullptr when there are none).
+typedef hipError_t (*SegBlockLaunch)(const void *tiles, uint64_t ntiles, int cin, int cout, const KSegH *kseg,
+                                     hipStream_t s);
 // part == nullptr: co-ranks found inside the merge kernel; else a table of
 // ceil(n / tile) entries that the partition kernel fills first.
 // `mid` (nullable) is recorded between the partition and the merge kernel.
@@ -134,11 +163,22 @@ hipError_t launch_generate_untag(int width, const void *pairs, uint64_t n, void 
 typedef hipError_t (*KwayLaunch)(const void *in, void *out, uint64_t n, int rlog2, int slog2, int cout,
                                  uint64_t *tab, const void *s0, const void *sm, const SampleSpec *next, hipStream_t s,
                                  hipEvent_t mid, uint32_t *err);
+// Pass `pass` of the K-way plan for many segments at once, all of fan-in KW
+// at that pass (the launch's kernel): tmap[i] = {segment index into segs, tile
+// index within the segment} for each of the launch's ntiles tiles; runs of
+// 2^rlog2 items and samples every 2^slog2 items in every listed segment; the
+// merged samples are in smp[smi].  The split kernel, then the merge kernel.
+typedef hipError_t (*SegKwayLaunch)(const KSegH *segs, const uint2 *tmap, uint64_t ntiles, int pass, int rlog2,
+                                    int slog2, int smi, uint32_t *err, hipStream_t s);
 struct KwayShapeFns {
     int block, ipt;
     KwayLaunch run[5];  // fan-in 2, 4, 8, 16, 32
+    SegKwayLaunch run_seg[5];
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
+// Writes tmap[tfirst[i] + k] = {list[i], k} for k < tfirst[i + 1] - tfirst[i].
+hipError_t launch_kway_tilemap(const uint32_t *list, const uint32_t *tfirst, uint32_t nlist, uint2 *tmap,
+                               hipStream_t s);
 // Tiles of one K-way pass (0 when the shape cannot hold kw slices at this
 // sample spacing).
 uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int ipt);

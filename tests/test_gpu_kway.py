@@ -223,3 +223,94 @@ def test_kway_default_tiles(ctx, kind, dist):
         check_kv(kind, gpu_sort(ctx, kind, a), a)
     else:
         check(kind, gpu_sort(ctx, kind, a), a)
+
+
+# Batched segments on the K-way plan: segments longer than one large-class
+# tile take K-way passes of fan-in kway_fanin (the last of a segment's passes
+# may be narrower), all segments of a pass in one split and one merge launch.
+BATCH_KINDS = (ogs.U32, ogs.F32, ogs.U64, ogs.I64, ogs.KV_U64U64, ogs.KV_I64I64, ogs.KV_U32U32)
+TILE2048 = {4: 10, 8: 1, 16: 0}  # a 2048-item block shape per item width (256 x 8)
+
+
+def batch_case(kind, dist, lens, seed):
+    import torch
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    n = int(off[-1])
+    d0 = ogs.NORMAL if kind == ogs.F32 and dist == ogs.UNIFORM else dist
+    a = ogs.generate(kind, d0, n, seed=seed)
+    return a, off, torch.from_numpy(off.view(np.int64).copy()).cuda()
+
+
+def check_segments(kind, got, a, off):
+    ref = oracle.segmented_sort(kind, a, off, threads=4)
+    if kind in (ogs.KV_U64U64, ogs.KV_I64I64, ogs.KV_U32U32):
+        assert np.array_equal(got["key"], ref["key"]), "key sequence differs"
+        seg = np.repeat(np.arange(off.size - 1), np.diff(off).astype(np.int64))
+        oi = np.lexsort((a["value"], a["key"], seg))
+        oo = np.lexsort((got["value"], got["key"], seg))
+        assert np.array_equal(a[oi].view(np.uint8), got[oo].view(np.uint8)), "values moved across keys"
+    else:
+        assert np.array_equal(got.view(np.uint8), ref.view(np.uint8))
+
+
+@pytest.mark.parametrize("kind", BATCH_KINDS)
+@pytest.mark.parametrize("fanin", [4, 16])
+@pytest.mark.parametrize("dist", [ogs.UNIFORM, ogs.ZIPF, ogs.SORTED, ogs.REVERSE, ogs.DISTINCT16])
+def test_batched_kway(ctx, kind, fanin, dist):
+    # 2048-item large-class tiles: with fan-in 4 a segment of 2^17 + 5 items
+    # takes four K-way passes, the last of fan-in 2
+    import torch
+    rng = np.random.default_rng(fanin * 7 + dist)
+    lens = np.floor(2.0 ** (rng.random(300) * 17)).astype(np.int64)
+    lens = np.concatenate([lens, [2049, 4096, 8192, 8193, 1 << 15, (1 << 15) + 3, 1 << 17, (1 << 17) + 5,
+                                  300001, 0, 1]])
+    rng.shuffle(lens)
+    a, off, doff = batch_case(kind, dist, lens, seed=fanin + 10 * dist)
+    n = a.size
+    say(f"case batched kway kind={ogs.KIND_NAMES[kind]} fanin={fanin} dist={dist} n={n} segs={lens.size}")
+    try:
+        ctx.tune_set(kind, "seg_large_block", TILE2048[ogs.WIDTH[kind]])
+        ctx.tune_set(kind, "kway_fanin", fanin)
+        ctx.tune_set(kind, "kway_sample", 5)
+        d = ogs.to_device(a)
+        s = torch.empty_like(d)
+        p = ctx.segmented_ptr(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+        torch.cuda.synchronize()
+        got = ogs.from_device(d if p == d.data_ptr() else s, kind, n)
+    finally:
+        ctx.tune_reset()
+    check_segments(kind, got, a, off)
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.I64, ogs.KV_U64U64, ogs.KV_U32U32])
+def test_batched_kway_default_tiles(ctx, kind):
+    # the tile table's own large-class plan, segments of up to 2^22 items,
+    # eager and as a replayed graph
+    import torch
+    ctx.tune_reset()
+    rng = np.random.default_rng(kind)
+    lens = np.floor(2.0 ** (8 + rng.random(60) * 14)).astype(np.int64)
+    lens = np.concatenate([lens, [1 << 22, (1 << 20) + 1, 65536, 65537]])
+    a, off, doff = batch_case(kind, ogs.ZIPF, lens, seed=kind)
+    n = a.size
+    say(f"case batched kway default tiles kind={ogs.KIND_NAMES[kind]} n={n}")
+    d = ogs.to_device(a)
+    s = torch.empty_like(d)
+    p = ctx.segmented_ptr(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+    torch.cuda.synchronize()
+    got = ogs.from_device(d if p == d.data_ptr() else s, kind, n)
+    check_segments(kind, got, a, off)
+    plan, res = ctx.segmented_plan(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+    try:
+        for rep in range(2):
+            d.copy_(ogs.to_device(a))
+            torch.cuda.synchronize()
+            plan.launch()
+            torch.cuda.synchronize()
+            g2 = ogs.from_device(d if res == d.data_ptr() else s, kind, n)
+            if kind in (ogs.KV_U64U64, ogs.KV_U32U32):
+                assert np.array_equal(g2["key"], got["key"])
+            else:
+                assert np.array_equal(g2.view(np.uint8), got.view(np.uint8))
+    finally:
+        plan.close()
