@@ -300,19 +300,53 @@ def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5, coalesce=0):
         reqs.append((kind, n, src))
     total_keys = sum(r[1] for r in reqs)
     log(f"bench: serving {nreq} requests, {total_keys} keys")
-    # serial service times (one plain launch per request) set the offered rate
+    # serial service times (one plain launch per request) set the offered rate;
+    # HIP events around each launch split a request's wall time into kernel
+    # time and the fixed host cost (issue and synchronisation)
     outs_serial = []
-    svc = []
+    svc, ker = [], []
     for kind, n, src in reqs:
         a_ = torch.empty_like(src)
         b_ = torch.empty_like(src)
         torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
+        e0.record()
         p = ctx.sort_keep_ptr(kind, src.data_ptr(), a_.data_ptr(), b_.data_ptr(), n)
+        e1.record()
         torch.cuda.synchronize()
         svc.append(time.perf_counter() - t0)
+        ker.append(e0.elapsed_time(e1) / 1e3)
         outs_serial.append(a_ if p == a_.data_ptr() else b_)
     mean_svc = float(np.mean(svc))
+    buckets = {}
+    for (kind, n, _), w_, k_ in zip(reqs, svc, ker):
+        b = "< 2^20" if n < (1 << 20) else ("2^20 .. 2^22" if n <= (1 << 22) else "> 2^22")
+        e = buckets.setdefault(b, {"requests": 0, "keys": 0, "wall_ms": 0.0, "kernel_ms": 0.0})
+        e["requests"] += 1
+        e["keys"] += n
+        e["wall_ms"] += w_ * 1e3
+        e["kernel_ms"] += k_ * 1e3
+    for e in buckets.values():
+        e["wall_ms"] = round(e["wall_ms"], 3)
+        e["kernel_ms"] = round(e["kernel_ms"], 3)
+    # the coalescing bound: every request of a key type in one batch, the
+    # types back to back (no arrivals to wait for)
+    bound_s = 0.0
+    for k in kinds:
+        sel = [(n, src) for kk, n, src in reqs if kk == k]
+        if not sel:
+            continue
+        outs_b = [torch.empty_like(src) for _, src in sel]
+        scr = [torch.empty_like(src) for _, src in sel]
+        items = [(src.data_ptr(), o.data_ptr(), sc.data_ptr(), n) for (n, src), o, sc in zip(sel, outs_b, scr)]
+        ctx.sort_batch_ptrs(k, items)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.sort_batch_ptrs(k, items)
+        torch.cuda.synchronize()
+        bound_s += time.perf_counter() - t0
+        del outs_b, scr
     rate = rate_factor / mean_svc
     gaps = rng.exponential(1.0 / rate, nreq)
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -364,6 +398,12 @@ def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5, coalesce=0):
                        "p99_ms": round(pct(lat_v, 99), 3), "batches": st["batches"]},
             "serial_plain": {"keys_per_s": round(total_keys / span_s / 1e9, 4), "p50_ms": round(pct(lat_s, 50), 3),
                              "p99_ms": round(pct(lat_s, 99), 3)},
+            "server_vs_serial": round(span_s / span_v, 3),
+            "serial_time_by_size": buckets,
+            "serial_wall_ms": round(sum(svc) * 1e3, 3), "serial_kernel_ms": round(sum(ker) * 1e3, 3),
+            "coalesced_bound": {"ms": round(bound_s * 1e3, 3), "keys_per_s": round(total_keys / bound_s / 1e9, 4),
+                                "vs_serial_wall": round(sum(svc) / bound_s, 3),
+                                "note": "each key type's requests in one ogs_sort_batch, types back to back"},
             "streams": streams, "unit": "G keys/s", "identical_results": bool(same),
             "note": "device-resident requests; latency = submit (arrival) to result ready"}
 
@@ -391,7 +431,7 @@ def main():
                     help="draw exactly this many segments instead (100000 = the literal C4 batch, ~7.6e9 keys)")
     ap.add_argument("--serving-requests", type=int, default=48)
     ap.add_argument("--serving-streams", type=int, default=4)
-    ap.add_argument("--serving-load", type=float, default=1.5,
+    ap.add_argument("--serving-load", type=float, default=3.0,
                     help="offered request rate as a multiple of the serial plain-launch capacity")
     ap.add_argument("--serving-coalesce", type=int, default=0,
                     help="longest request the server batches with others (0: the server's default)")

@@ -273,16 +273,19 @@ int ogs_profile_reset(ogs_ctx *ctx);
  * holds the batch's footprint (device request: one scratch copy; host
  * request: staging + scratch = 2x payload), so requests arriving while
  * others run are admitted as capacity frees up and their work overlaps on
- * the device.  A completion thread finishes batches (host requests are
- * copied back) and timestamps every request.  No reference counterpart: the
- * reference is a library call (Origami/sorter.h:3851, 4194). */
+ * the device.  The batch's key type is the oldest request's, unless a batch of
+ * that type is already running and a request of a type with none waits among
+ * the next 64 (then that type goes first).  Each stream has a waiter thread
+ * that finishes its batch (host requests are copied back) and timestamps the
+ * requests.  No reference counterpart: the reference is a library call
+ * (Origami/sorter.h:3851, 4194). */
 typedef struct ogs_server ogs_server;
 typedef struct {
     uint64_t hbm_budget_bytes;   /* device arena size; admission limit */
     int streams;                 /* batches in flight at once (1..16) */
     uint32_t max_batch_requests; /* 0 = 1024 */
     uint64_t max_batch_items;    /* 0 = unlimited */
-    uint64_t coalesce_max_items; /* longer requests run as batches of one; 0 = 2^20 */
+    uint64_t coalesce_max_items; /* longer requests run alone as plain sorts; 0 = 2^22 */
 } ogs_server_config;
 typedef struct {
     uint64_t submitted, completed, rejected, batches, batched_requests, items;

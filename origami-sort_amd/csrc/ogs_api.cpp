@@ -138,6 +138,15 @@ Workspace *Ctx::workspace(hipStream_t s) {
     return w;
 }
 
+// Frees stream s's workspace (the caller has finished all work on s).
+void Ctx::drop_workspace(hipStream_t s) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = ws.find(s);
+    if (it == ws.end()) return;
+    delete it->second;
+    ws.erase(it);
+}
+
 int Workspace::reserve(size_t bytes) {
     if (bytes <= cap) return OGS_OK;
     size_t want = std::max(bytes, cap * 2);

@@ -183,6 +183,7 @@ struct Ctx {
     size_t host_cap = 0;
 
     Workspace *workspace(hipStream_t s);
+    void drop_workspace(hipStream_t s);
     const BlockShapeFns &block_shape(KeyClass kc, int param);
     const MergeShapeFns &merge_shape(KeyClass kc, int param, uint64_t block_tile);
     // in == nullptr: sort data in place (data/scratch ping-pong); otherwise read

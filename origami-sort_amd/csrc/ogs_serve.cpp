@@ -4,17 +4,24 @@
 // form of its sort entry points (Origami/sorter.h:3851, 4194).  Requests are
 // ORDER-BY-style sorts of device or host arrays of any supported key type.
 //
-//   submit ──► pending FIFO ──► scheduler thread ──► batch on a free stream ──► completion thread
+//   submit ──► pending FIFO ──► scheduler thread ──► batch on a free stream ──► the stream's waiter
 //                               (admission by HBM      (ogs_sort_batch over       (event wait, host
 //                                footprint, pooled      the batch's requests)      copy-back, release)
 //                                device arena)
 //
-// Admission: a batch takes pending requests of the head's key type in FIFO
-// order while their footprint fits the arena (device request: one scratch
-// copy of the payload; host request: payload staging + scratch = 2x) and the
-// batch caps allow.  Batches run on `streams` HIP streams, so requests that
-// arrive while others are on the GPU are admitted as soon as a stream and
-// arena space free up, and their kernels overlap with the running batches.
+// Admission: a batch takes pending requests of one key type in FIFO order
+// while their footprint fits the arena (device request: one scratch copy of
+// the payload; host request: payload staging + scratch = 2x) and the batch
+// caps allow.  The key type is the oldest request's, unless a batch of that
+// type is already on the GPU and a request of a type with none waits among
+// the next 64: then that type goes first, so the types overlap on the GPU
+// while requests of a running type gather into its next batch.  Requests
+// longer than coalesce_max run alone as plain sorts.  Batches run on
+// `streams` HIP streams, one batch per stream at a time, and each stream has
+// its own waiter thread, so a stream is free again as soon as its own batch
+// ends (not when an older batch on another stream does); requests that arrive
+// while others are on the GPU are admitted as soon as a stream and arena space
+// free up, and their kernels overlap with the running batches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -106,6 +113,7 @@ struct Batch {
     int stream = -1;
     hipEvent_t ev = nullptr;
     int status = OGS_OK;
+    bool issued = false;  // launches done and the event recorded (its waiter may wait on it)
 };
 
 }  // namespace
@@ -116,18 +124,19 @@ struct Server {
     void *arena_base = nullptr;
     Arena arena;
     std::vector<hipStream_t> streams;
-    std::vector<bool> stream_busy;
+    std::vector<Batch *> slot;  // per stream: its batch on the GPU, or nullptr
     std::mutex mu;
-    std::condition_variable cv_sched, cv_done, cv_inflight;
+    std::condition_variable cv_sched, cv_done, cv_slot;
     std::deque<Request *> pending;
     std::map<uint64_t, Request *> reqs;
-    std::deque<Batch *> inflight;
-    std::thread sched, completer;
+    std::map<int, int> running;  // key type -> batches of it on the GPU
+    std::thread sched;
+    std::vector<std::thread> waiters;
     bool stopping = false;     // no new submissions; drain what is queued
     bool sched_done = false;   // the scheduler has issued its last batch
     uint64_t releases = 0;     // arena releases so far (admission retries on change)
     uint64_t next_ticket = 1;
-    uint64_t coalesce_max = 1u << 20;  // longest request that is batched with others
+    uint64_t coalesce_max = 1u << 22;  // longest request that is batched with others
     double t0 = 0;
     ogs_server_stats stats{};
 
@@ -137,8 +146,13 @@ struct Server {
         return (r->flags & OGS_SUBMIT_HOST) ? 2 * one : one;
     }
 
+    int free_stream() const {
+        for (size_t i = 0; i < slot.size(); i++)
+            if (!slot[i]) return (int)i;
+        return -1;
+    }
     void schedule_loop();
-    void complete_loop();
+    void wait_loop(int si);
     void finish(Batch *b);
 };
 
@@ -149,17 +163,26 @@ void Server::schedule_loop() {
         cv_sched.wait(lk, [&] {
             if (stopping && pending.empty()) return true;
             if (pending.empty()) return false;
-            return std::find(stream_busy.begin(), stream_busy.end(), false) != stream_busy.end();
+            return free_stream() >= 0;
         });
         if (pending.empty()) {
             if (stopping) break;
             continue;
         }
-        const int si = (int)(std::find(stream_busy.begin(), stream_busy.end(), false) - stream_busy.begin());
-        // coalesce: FIFO requests of the head's key type, within caps and arena
+        const int si = free_stream();
+        // the batch's key type: the oldest request's, unless that type is on
+        // the GPU and one that is not waits close behind
         Request *head = pending.front();
+        if (running[head->kt] > 0) {
+            size_t k = 0;
+            for (auto it = pending.begin(); it != pending.end() && k < 64; ++it, ++k)
+                if (running[(*it)->kt] == 0) {
+                    head = *it;
+                    break;
+                }
+        }
         if (footprint(head) > arena.capacity()) {
-            pending.pop_front();
+            pending.erase(std::find(pending.begin(), pending.end(), head));
             head->status = fail(OGS_ERR_NOMEM, "request of %llu items exceeds the server's HBM budget",
                                 (unsigned long long)head->n);
             head->done = true;
@@ -175,10 +198,15 @@ void Server::schedule_loop() {
         // long requests run alone (a batch finishes when its last request
         // does, so coalescing them would only delay the short ones)
         const bool alone = head->n > coalesce_max;
-        for (auto it = pending.begin(); it != pending.end() && scanned < 256; ++it, ++scanned) {
+        if (alone) {
+            head->dev_off = 0;
+            take.push_back(head);
+            bytes = footprint(head);
+            items = head->n;
+        }
+        for (auto it = pending.begin(); !alone && it != pending.end() && scanned < 256; ++it, ++scanned) {
             Request *r = *it;
-            if (r->kt != head->kt || (r != head && (alone || r->n > coalesce_max))) continue;
-            if (alone && !take.empty()) break;
+            if (r->kt != head->kt || r->n > coalesce_max) continue;
             const size_t f = footprint(r);
             if (!take.empty() && (take.size() >= cfg.max_batch_requests || items + r->n > cfg.max_batch_items ||
                                   bytes + f > arena.capacity()))
@@ -206,7 +234,8 @@ void Server::schedule_loop() {
         b->reqs = take;
         b->arena_off = off;
         b->stream = si;
-        stream_busy[si] = true;
+        slot[si] = b;
+        running[head->kt]++;
         const double ts = now_s() - t0;
         for (Request *r : take) r->t_start = ts;
         lk.unlock();
@@ -251,8 +280,8 @@ void Server::schedule_loop() {
         lk.lock();
         stats.batches++;
         stats.batched_requests += take.size();
-        inflight.push_back(b);
-        cv_inflight.notify_all();
+        b->issued = true;
+        cv_slot.notify_all();
     }
 }
 
@@ -278,40 +307,26 @@ void Server::finish(Batch *b) {
     }
     arena.release(b->arena_off);
     releases++;
-    stream_busy[b->stream] = false;
+    running[b->reqs.front()->kt]--;
+    slot[b->stream] = nullptr;
     if (b->ev) (void)hipEventDestroy(b->ev);
     delete b;
     cv_done.notify_all();
     cv_sched.notify_all();
 }
 
-void Server::complete_loop() {
+// Stream si's waiter: waits for each batch issued on the stream, then
+// finishes it (results copied back, requests done, stream and arena freed).
+void Server::wait_loop(int si) {
     (void)hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-        cv_inflight.wait(lk, [&] { return !inflight.empty() || sched_done; });
-        if (inflight.empty()) {
-            if (sched_done) break;
-            continue;
-        }
-        // finish whichever batch is ready first; else block on the oldest
-        Batch *ready = nullptr;
-        for (auto it = inflight.begin(); it != inflight.end(); ++it) {
-            if (!(*it)->ev || hipEventQuery((*it)->ev) != hipErrorNotReady) {
-                ready = *it;
-                inflight.erase(it);
-                break;
-            }
-        }
-        if (!ready) {
-            Batch *oldest = inflight.front();
-            lk.unlock();
-            (void)hipEventSynchronize(oldest->ev);
-            lk.lock();
-            continue;
-        }
+        cv_slot.wait(lk, [&] { return (slot[si] && slot[si]->issued) || sched_done; });
+        if (!slot[si] || !slot[si]->issued) break;  // sched_done and nothing left on this stream
+        Batch *b = slot[si];
         lk.unlock();
-        finish(ready);
+        if (b->ev) (void)hipEventSynchronize(b->ev);
+        finish(b);
         lk.lock();
     }
 }
@@ -341,14 +356,37 @@ int ogs_server_create(ogs_server **srv, ogs_ctx *ctx, const ogs_server_config *c
     }
     s->arena.init(cfg->hbm_budget_bytes);
     s->streams.resize(cfg->streams);
-    s->stream_busy.assign(cfg->streams, false);
-    for (auto &st : s->streams) {
+    s->slot.assign(cfg->streams, nullptr);
+    // Each stream's working memory (sample arrays and split tables of its
+    // sorts, batch descriptor tables, co-rank tables) is reserved now, at
+    // budget / 8 per stream within [64 MiB, 1 GiB]: growing it while batches
+    // run would free the old block, and hipFree waits for the whole device.
+    const size_t pre = std::min<size_t>(std::max<size_t>(cfg->hbm_budget_bytes / 8, 64u << 20), 1u << 30);
+    int rc = OGS_OK;
+    size_t made = 0;
+    for (; made < s->streams.size() && rc == OGS_OK; made++) {
+        hipStream_t &st = s->streams[made];
         e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-        if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+        if (e != hipSuccess) {
+            rc = hip_fail(e, "hipStreamCreate");
+            break;
+        }
+        Workspace *w = s->ctx->workspace(st);
+        rc = w->reserve(pre);
+        if (rc == OGS_OK) rc = w->reserve_part(16u << 20);
+    }
+    if (rc != OGS_OK) {  // undo: the streams made so far, their workspaces, the arena
+        for (size_t i = 0; i < made; i++) {
+            s->ctx->drop_workspace(s->streams[i]);
+            (void)hipStreamDestroy(s->streams[i]);
+        }
+        (void)hipFree(s->arena_base);
+        delete s;
+        return rc;
     }
     s->t0 = now_s();
     s->sched = std::thread([s] { s->schedule_loop(); });
-    s->completer = std::thread([s] { s->complete_loop(); });
+    for (int i = 0; i < cfg->streams; i++) s->waiters.emplace_back([s, i] { s->wait_loop(i); });
     *srv = reinterpret_cast<ogs_server *>(s);
     return OGS_OK;
 }
@@ -361,15 +399,18 @@ int ogs_server_destroy(ogs_server *srv) {
         s->stopping = true;
     }
     s->cv_sched.notify_all();
-    s->cv_inflight.notify_all();
     s->sched.join();
     {
         std::lock_guard<std::mutex> g(s->mu);
         s->sched_done = true;
     }
-    s->cv_inflight.notify_all();
-    s->completer.join();
-    for (auto &st : s->streams) (void)hipStreamDestroy(st);
+    s->cv_slot.notify_all();
+    for (auto &w : s->waiters) w.join();
+    for (auto &st : s->streams) {
+        (void)hipStreamSynchronize(st);
+        s->ctx->drop_workspace(st);
+        (void)hipStreamDestroy(st);
+    }
     (void)hipFree(s->arena_base);
     for (auto &kv : s->reqs) delete kv.second;
     delete s;

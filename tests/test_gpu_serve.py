@@ -1,6 +1,8 @@
 """Serving loop: every request's result in a concurrent mix equals the oracle's
-and the same request sorted alone; host and device requests; admission under
-a tight HBM budget; oversize requests are rejected with a status."""
+and the same request sorted alone; requests up to 2^24 keys, which run alone
+as plain sorts beside the coalesced batches; host and device requests;
+admission under a tight HBM budget; oversize requests are rejected with a
+status."""
 import sys
 
 import numpy as np
@@ -27,7 +29,7 @@ def ctx():
 
 
 def _check(kind, got, a):
-    ref = oracle.sort(kind, a)
+    ref = oracle.sort(kind, a, threads=8)
     if kind in (ogs.KV_U64U64, ogs.KV_I64I64):
         assert np.array_equal(got["key"], ref["key"])
         assert np.array_equal(np.sort(got["value"]), np.sort(a["value"]))
@@ -98,3 +100,82 @@ def test_host_requests_and_budget(ctx):
     with pytest.raises(ogs.OgsError):
         srv.wait(t, 30.0)
     srv.close()
+
+
+def _serve_and_check(ctx, reqs, streams, coalesce=0, budget=None, gaps_ms=None):
+    """Submits every (kind, n, host array) request (with the given gaps between
+    submissions), waits, and checks each result against the oracle and against
+    the same request sorted alone by the plain entry point."""
+    import time
+
+    import torch
+    bufs = []
+    for kind, n, a in reqs:
+        src = ogs.to_device(a)
+        bufs.append((src, torch.empty_like(src)))
+    need = sum(n * ogs.WIDTH[k] for k, n, _ in reqs)
+    srv = ogs.Server(ctx, hbm_budget_bytes=int(budget or need + (64 << 20)), streams=streams,
+                     coalesce_max_items=coalesce)
+    torch.cuda.synchronize()
+    tickets = []
+    try:
+        for j, ((kind, n, a), (src, dst)) in enumerate(zip(reqs, bufs)):
+            if gaps_ms is not None:
+                time.sleep(gaps_ms[j] / 1e3)
+            tickets.append(srv.submit(kind, src.data_ptr(), dst.data_ptr(), n))
+        for t in tickets:
+            srv.wait(t, 120.0)
+        st = srv.stats()
+    finally:
+        srv.close()
+    say(f"stats {st}")
+    assert st["completed"] == len(reqs)
+    for (kind, n, a), (src, dst) in zip(reqs, bufs):
+        say(f"check kind={ogs.KIND_NAMES[kind]} n={n}")
+        got = ogs.from_device(dst, kind, n)
+        _check(kind, got, a)
+        if kind not in (ogs.KV_U64U64, ogs.KV_I64I64):
+            d = ogs.to_device(a)
+            s = torch.empty_like(d)
+            res = ctx.sort_tensor(kind, d, s, n)
+            torch.cuda.synchronize()
+            assert np.array_equal(ogs.from_device(res, kind, n).view(np.uint8), got.view(np.uint8))
+    return st
+
+
+def test_large_requests_beside_batches(ctx):
+    # requests above the coalescing limit (2^22) up to 2^24 keys run alone as
+    # plain sorts on their own streams while the short ones are coalesced
+    rng = np.random.default_rng(33)
+    kinds = [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.F32, ogs.I64]
+    sizes = [1 << 24, (1 << 23) + 5, (1 << 22) + 7, 1 << 22, (1 << 22) + 1, 3000001]
+    sizes += [int(2 ** rng.uniform(10, 20)) for _ in range(30)]
+    reqs = []
+    for j, n in enumerate(sizes):
+        kind = kinds[j % len(kinds)] if n < (1 << 24) else ogs.U32
+        if kind == ogs.KV_U64U64 and n > (1 << 23):
+            kind = ogs.U64
+        dist = ogs.NORMAL if kind == ogs.F32 else (ogs.ZIPF if j % 2 else ogs.UNIFORM)
+        reqs.append((kind, n, ogs.generate(kind, dist, n, seed=700 + j)))
+    order = rng.permutation(len(reqs))
+    reqs = [reqs[i] for i in order]
+    say(f"case serve large: {len(reqs)} requests up to 2^24 keys, 4 streams, one burst")
+    st = _serve_and_check(ctx, reqs, streams=4)
+    assert st["batches"] < len(reqs)  # the short ones were coalesced
+    # the same requests arriving spread out (Poisson gaps of 0.2 ms on average)
+    say("case serve large: spread arrivals")
+    _serve_and_check(ctx, reqs, streams=4, gaps_ms=rng.exponential(0.2, len(reqs)))
+
+
+def test_every_request_alone_on_many_streams(ctx):
+    # coalescing off (limit 1000 items): every longer request is its own plain
+    # sort, up to 8 of them on the GPU at once
+    rng = np.random.default_rng(34)
+    kinds = [ogs.U32, ogs.KV_U64U64, ogs.U64, ogs.I32]
+    reqs = []
+    for j in range(24):
+        kind = kinds[j % len(kinds)]
+        n = int(2 ** rng.uniform(8, 21))
+        reqs.append((kind, n, ogs.generate(kind, ogs.ZIPF if j % 3 else ogs.UNIFORM, n, seed=900 + j)))
+    say("case serve alone: 24 requests, coalescing off, 8 streams")
+    _serve_and_check(ctx, reqs, streams=8, coalesce=1000)
