@@ -272,15 +272,73 @@ template <int KW> __device__ __forceinline__ uint64_t lane_scan64(uint64_t x) {
     return x;
 }
 
-// Items tid + k * BLOCK (k < IPT) of an LDS tile (sp0 = its swizzled base) to
-// ob[tid + k * BLOCK], with output codec CODEC: all reads, then all stores.
+// Items [0, cnt) of a merged LDS tile (position p is output item p of ob) to
+// ob with 16-byte stores, applying the output codec CODEC: chunk c = tid + k *
+// BLOCK (k < IPT / E, E = 16 / sizeof(K)) is LDS positions E*c .. E*c + E - 1,
+// one aligned 16-byte LDS read (the swizzle keeps a 16-byte chunk whole, and
+// lane-consecutive chunks hit distinct banks), and one 16-byte global store to
+// ob + E*c, whose address is aligned to the item size (global vector accesses
+// need dword alignment only).  The last cnt % E items go one per thread, and so
+// does every item of a buffer that is not aligned to its item size.
 template <int BLOCK, int IPT, int CODEC, typename K>
-__device__ __forceinline__ void store_items(const K *sp0, K *ob, int tid) {
+__device__ __forceinline__ void store_chunks(const K *sk, K *ob, int cnt, int tid) {
+    constexpr int E = chunk_items<K>();
+    constexpr int NK = IPT / E;
+    static_assert(IPT % E == 0 && (E * BLOCK) % (sizeof(K) == 4 ? 256 : 128) == 0, "chunks: whole swizzle periods");
+    if (((uintptr_t)ob & (sizeof(K) - 1)) != 0) {
+        // a buffer not aligned to its items (uniform): one item per store
+#pragma unroll 4
+        for (int p = tid; p < cnt; p += BLOCK) ob[p] = dec_key(sk[lsw<K>(p)], CODEC);
+        return;
+    }
+    const int nb = cnt / E;
+    const int tail = cnt - nb * E;
+    if (tid < tail) {
+        const int p = nb * E + tid;
+        ob[p] = dec_key(sk[lsw<K>(p)], CODEC);
+    }
+    // chunk k's LDS address is the first chunk's plus k * E * BLOCK (a
+    // multiple of the swizzle period)
+    const K *s0 = sk + lsw<K>(E * tid);
+    const CodecMask<K> m = codec_mask<K>(CODEC);
+    auto put = [&](int k, const K *x) {
+        K *d = ob + (uint32_t)(E * (tid + k * BLOCK));
+        if constexpr (E == 4)
+            *(uint4 *)d = uint4{(uint32_t)dec_bf(x[0], m), (uint32_t)dec_bf(x[1], m), (uint32_t)dec_bf(x[2], m),
+                                (uint32_t)dec_bf(x[3], m)};
+        else
+            *(ulonglong2 *)d = ulonglong2{(unsigned long long)dec_bf(x[0], m), (unsigned long long)dec_bf(x[1], m)};
+    };
     K v[IPT];
+    if (tid + (NK - 1) * BLOCK < nb) {
+        // every chunk of this thread is whole: all reads, then all stores
 #pragma unroll
-    for (int k = 0; k < IPT; k++) v[k] = dec_key(sp0[k * BLOCK], CODEC);
+        for (int k = 0; k < NK; k++) {
+            if constexpr (E == 4) {
+                const uint4 q = *(const uint4 *)(s0 + k * E * BLOCK);
+                v[4 * k] = q.x, v[4 * k + 1] = q.y, v[4 * k + 2] = q.z, v[4 * k + 3] = q.w;
+            } else {
+                const ulonglong2 q = *(const ulonglong2 *)(s0 + k * E * BLOCK);
+                v[2 * k] = (K)q.x, v[2 * k + 1] = (K)q.y;
+            }
+        }
 #pragma unroll
-    for (int k = 0; k < IPT; k++) ob[(uint32_t)(tid + k * BLOCK)] = v[k];
+        for (int k = 0; k < NK; k++) put(k, v + E * k);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NK; k++) {
+            if (tid + k * BLOCK < nb) {
+                if constexpr (E == 4) {
+                    const uint4 q = *(const uint4 *)(s0 + k * E * BLOCK);
+                    v[0] = q.x, v[1] = q.y, v[2] = q.z, v[3] = q.w;
+                } else {
+                    const ulonglong2 q = *(const ulonglong2 *)(s0 + k * E * BLOCK);
+                    v[0] = (K)q.x, v[1] = (K)q.y;
+                }
+                put(k, v);
+            }
+        }
+    }
 }
 
 // ----------------------------------------------------------------------------
@@ -576,8 +634,8 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
     // it as their base and a 32-bit lane offset
     const uint64_t o0u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(o0 >> 32)) << 32) |
                          (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o0);
-    const K *sp0 = sk + lsw<K>(tid);
     if constexpr (HV) {
+        const K *sp0 = sk + lsw<K>(tid);
         ulonglong2 *ob = (ulonglong2 *)out + o0u;
         const uint64_t *vp0 = sv + lsw<uint64_t>(tid);
         const CodecMask<K> m = codec_mask<K>(cout);
@@ -592,18 +650,10 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
         }
     } else {
         K *ob = (K *)out + o0u;
-        if (tid + (IPT - 1) * BLOCK < cnt) {
-            // all IPT items of this thread are real: every read, then every
-            // store (the codec is uniform: one branch picks the loop)
-            if (cout == 0) store_items<BLOCK, IPT, 0>(sp0, ob, tid);
-            else if (cout == 1) store_items<BLOCK, IPT, 1>(sp0, ob, tid);
-            else store_items<BLOCK, IPT, sizeof(K) == 4 ? 2 : 3>(sp0, ob, tid);
-        } else {
-            const int nk = cnt > tid ? (cnt - tid + BLOCK - 1) / BLOCK : 0;
-#pragma unroll
-            for (int k = 0; k < IPT; k++)
-                if (k < nk) ob[(uint32_t)(tid + k * BLOCK)] = dec_key(sp0[k * BLOCK], cout);
-        }
+        // the codec is uniform: one branch picks the store loop
+        if (cout == 0) store_chunks<BLOCK, IPT, 0>(sk, ob, cnt, tid);
+        else if (cout == 1) store_chunks<BLOCK, IPT, 1>(sk, ob, cnt, tid);
+        else store_chunks<BLOCK, IPT, sizeof(K) == 4 ? 2 : 3>(sk, ob, cnt, tid);
     }
     if (sp.smp) write_samples<BLOCK>(sk, cnt, o0, sp.smp, sp.rlog2, sp.kw, sp.slog2);
 }
