@@ -50,8 +50,10 @@ def traffic_from_profiles(kernel, workload):
 
 
 def host_cpu():
-    """CPU model, physical cores and logical CPUs of this host (/proc/cpuinfo)."""
+    """CPU model (name, family/model numbers, AVX-512 presence), physical cores and
+    logical CPUs of this host (/proc/cpuinfo)."""
     model, phys = "unknown", set()
+    first = {}
     logical = os.cpu_count() or 1
     cur = {}
     try:
@@ -60,14 +62,20 @@ def host_cpu():
                 if not line.strip():
                     if "core id" in cur:
                         phys.add((cur.get("physical id", "0"), cur["core id"]))
+                    if not first:
+                        first = dict(cur)
                     cur = {}
                     continue
                 k, _, v = line.partition(":")
                 cur[k.strip()] = v.strip()
-                if k.strip() == "model name" and model == "unknown":
-                    model = v.strip()
     except OSError:
         pass
+    if first.get("model name"):
+        model = first["model name"]
+        if "cpu family" in first and "model" in first:
+            model += f" (family {first['cpu family']}, model {first['model']})"
+        flags = first.get("flags", "").split()
+        model += ", AVX-512" if "avx512f" in flags else (", AVX2" if "avx2" in flags else "")
     return model, (len(phys) or logical), logical
 
 
