@@ -229,7 +229,8 @@ int ogs_generate(ogs_ctx *ctx, ogs_key_t kt, int dist, uint64_t seed, uint64_t n
  * off; structure-of-arrays KV sorts keep the 2- and 4-way passes), "kway_shape" (index into ogs_shape_info's K-way
  * list), "kway_sample" (log2 of its sample spacing, 3-10), "kway_scorank" (its
  * sample merges: 0 a co-rank table kernel per merge, 1 co-ranks searched inside
- * the merge kernel).  Block, merge and
+ * the merge kernel), "kway_swhole" (1: when a group's samples fit one K-way
+ * tile, a pass merges them in one K-way launch; 0: 2-way merges).  Block, merge and
  * K-way shape parameters take an index into ogs_shape_info's lists; *_max
  * take an item count.  Segments of up to
  * seg_tiny_max items (0 = no tiny class) get one workgroup of the tiny shape

@@ -387,8 +387,9 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
         uint64_t lo = 0;
         if (j < KW) {
             const uint64_t lenj = kw_run_len(n, gbase, R, j);
-            lo = tab[t * KW + j];
-            const uint64_t hi = tl + 1 < nt_here ? tab[(t + 1) * KW + j] : lenj;
+            // no table: one tile per group, every run from its start
+            lo = tab ? tab[t * KW + j] : 0;
+            const uint64_t hi = tab && tl + 1 < nt_here ? tab[(t + 1) * KW + j] : lenj;
             s_src[j] = gbase + (uint64_t)j * R + lo;
             // a split table that is not monotone gives a negative count: the
             // tile is not merged and the error word reports it
@@ -688,6 +689,23 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     return hipGetLastError();
 }
 
+// Every group of KW runs of 2^rlog2 items (ragged at n) merged by one
+// workgroup, with no split table: the group, its slices padded to IPT, must fit
+// the BLOCK * IPT items of LDS.  Merges the samples of a K-way pass when a
+// group's samples fit one tile.
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
+static hipError_t launch_kway_whole(const void *in, void *out, uint64_t n, int rlog2, uint32_t *err, hipStream_t s) {
+    constexpr uint64_t C = (uint64_t)BLOCK * IPT;
+    if (n == 0) return hipSuccess;
+    const uint64_t R = 1ull << rlog2;
+    if (rlog2 > 30 || (uint64_t)KW * ((R + IPT - 1) / IPT * IPT) > C) return hipErrorInvalidValue;
+    const uint64_t groups = (n + R * KW - 1) / (R * KW);
+    const SampleSpec none = {nullptr, 0, 0, 0};
+    hipLaunchKernelGGL((k_kway_merge<K, HV, KW, BLOCK, IPT>), dim3((unsigned)groups), dim3(BLOCK), 0, s, in, out, n,
+                       rlog2, (uint64_t)1, (uint64_t)1, 0, (const uint64_t *)nullptr, none, err);
+    return hipGetLastError();
+}
+
 // ----------------------------------------------------------------------------
 // Batched form: one pass of many segments' K-way plans (segments of a batched
 // sort), all of fan-in KW at this pass.  Runs are 2^rlog2 items in every
@@ -763,7 +781,10 @@ static hipError_t launch_seg_kway(const KSegH *segs, const uint2 *tmap, uint64_t
     {B, I, {&launch_kway<K, HV, 2, B, I>, &launch_kway<K, HV, 4, B, I>, &launch_kway<K, HV, 8, B, I>,       \
             &launch_kway<K, HV, 16, B, I>, &launch_kway<K, HV, 32, B, I>},                                  \
      {&launch_seg_kway<K, HV, 2, B, I>, &launch_seg_kway<K, HV, 4, B, I>, &launch_seg_kway<K, HV, 8, B, I>, \
-      &launch_seg_kway<K, HV, 16, B, I>, &launch_seg_kway<K, HV, 32, B, I>}},
+      &launch_seg_kway<K, HV, 16, B, I>, &launch_seg_kway<K, HV, 32, B, I>},                                \
+     {&launch_kway_whole<K, HV, 2, B, I>, &launch_kway_whole<K, HV, 4, B, I>,                               \
+      &launch_kway_whole<K, HV, 8, B, I>, &launch_kway_whole<K, HV, 16, B, I>,                              \
+      &launch_kway_whole<K, HV, 32, B, I>}},
 
 static const KwayShapeFns kKway32[] = {
 #define X(B, I) OGS_KX(uint32_t, false, B, I)

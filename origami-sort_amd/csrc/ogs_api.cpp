@@ -83,6 +83,20 @@ static int kway_index(KeyClass kc, int block, int ipt) {
 // tuned constants of Origami/config.h:45-264).  A context starts from it;
 // ogs_tune_set / ogs_tune_load ($OGS_TILES, written by autotune.py) override
 // entries.
+// The K-way shape of the sample class skc whose one tile holds a whole group of
+// kw sample runs of 2^rs_log2 records (each padded to the shape's IPT), the
+// smallest such; nullptr if none does.
+static const KwayShapeFns *whole_group_shape(KeyClass skc, int kw, int rs_log2) {
+    int cnt = 0;
+    const KwayShapeFns *t = kway_shapes(skc, &cnt);
+    const KwayShapeFns *best = nullptr;
+    for (int j = 0; j < cnt; j++) {
+        const uint64_t ipt = (uint64_t)t[j].ipt, need = (uint64_t)kw * (((1ull << rs_log2) + ipt - 1) / ipt * ipt);
+        if (need <= t[j].tile() && (!best || t[j].tile() < best->tile())) best = &t[j];
+    }
+    return best;
+}
+
 Tuning default_tuning(KeyClass kc) {
     Tuning t;
     t.v[TP_PLAIN_CORANK] = 0;
@@ -92,6 +106,7 @@ Tuning default_tuning(KeyClass kc) {
     t.v[TP_PARTITION_LANES] = 4;
     t.v[TP_QUAD_LANES] = 1;
     t.v[TP_KWAY_SCORANK] = 1;
+    t.v[TP_KWAY_SWHOLE] = 1;
     if (kc == KC_32) {
         t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 32, 4);
         t.v[TP_PLAIN_MERGE] = merge_index(kc, 512, 32);
@@ -478,20 +493,27 @@ int Ctx::sort_kway(const KeyInfo &ki, const void *in, void *data, void *scratch,
         // 32-bit keys: a u64 sample is its own total order (key, run, index)
         // and unique, so when a group's samples fill exactly one block-sort
         // tile of the 64-bit class, one block sort replaces the merge passes
+        // a group's samples in one K-way tile: one launch merges every group
+        const KwayShapeFns *kwh =
+            tune[ki.kc].v[TP_KWAY_SWHOLE] ? whole_group_shape(skc, 1 << lv[p], rl - sl[p]) : nullptr;
         const BlockShapeFns *sbs = nullptr;
-        if (ki.kc == KC_32) {
+        if (ki.kc == KC_32 && !kwh) {
             const uint64_t gspan = 1ull << (rl - sl[p] + lv[p]);
             int bcnt = 0;
             const BlockShapeFns *bt = block_shapes(KC_64, &bcnt);
             for (int j = 0; j < bcnt && !sbs; j++)
                 if (bt[j].tile() == gspan) sbs = &bt[j];
         }
-        if (sbs) {
+        if (kwh) {
+            e = kwh->whole[lv[p] - 1](S0, S1, ns, rl - sl[p], err, s);
+            if (e != hipSuccess) return hip_fail(e, "sample merge launch");
+            sa = S1;
+        } else if (sbs) {
             e = sbs->run(S0, S1, ns, 0, 0, s, nullptr, nullptr, nullptr);
             if (e != hipSuccess) return hip_fail(e, "sample sort launch");
             sa = S1;
         }
-        for (int q = 0; q < (sbs ? 0 : lv[p]); q++) {
+        for (int q = 0; q < (sbs || kwh ? 0 : lv[p]); q++) {
             const uint64_t Rs = 1ull << (rl - sl[p] + q);
             int best = -1;
             for (int j = 0; j < mcnt; j++)

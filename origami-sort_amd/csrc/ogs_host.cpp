@@ -29,7 +29,7 @@ const char *kParamNames[TP_COUNT] = {"plain_block",     "plain_merge", "seg_smal
                                      "seg_small_max",   "seg_large_block", "seg_merge",
                                      "seg_tiny_block",  "seg_tiny_max", "plain_corank", "seg_plain_min",
                                      "partition_lanes", "plain_fanin", "quad_lanes", "kway_fanin",
-                                     "kway_shape", "kway_sample", "kway_scorank"};
+                                     "kway_shape", "kway_sample", "kway_scorank", "kway_swhole"};
 
 int param_index(const char *name) {
     if (!name) return -1;
@@ -225,7 +225,7 @@ static int tune_set_into(Tuning *t, const KeyInfo &ki, const char *param, int va
     const bool is_count = p == TP_SEG_SMALL_MAX || p == TP_SEG_TINY_MAX || p == TP_SEG_PLAIN_MIN;
     bool ok = true;
     if (p == TP_PLAIN_MERGE || p == TP_SEG_MERGE) merge_shapes(ki.kc, &cnt);
-    else if (p == TP_PLAIN_CORANK || p == TP_KWAY_SCORANK) cnt = 2;
+    else if (p == TP_PLAIN_CORANK || p == TP_KWAY_SCORANK || p == TP_KWAY_SWHOLE) cnt = 2;
     else if (p == TP_PLAIN_FANIN) ok = value == 2 || value == 4;
     else if (p == TP_QUAD_LANES) ok = value == 1 || value == 2 || value == 4 || value == 8;
     else if (p == TP_KWAY_FANIN) ok = value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32;

@@ -45,7 +45,8 @@ enum TuneParam {
     TP_KWAY_SHAPE = 14,     // tile shape of the K-way merge kernel
     TP_KWAY_SAMPLE = 15,    // log2 of the sample spacing of the K-way pass
     TP_KWAY_SCORANK = 16,   // sample merges of the K-way pass: 0 co-rank table kernel, 1 co-ranks inside the merge kernel
-    TP_COUNT = 17
+    TP_KWAY_SWHOLE = 17,    // 1: a pass's sample merge is one K-way launch when a group's samples fit one tile
+    TP_COUNT = 18
 };
 struct Tuning {
     int v[TP_COUNT];

@@ -170,10 +170,15 @@ typedef hipError_t (*KwayLaunch)(const void *in, void *out, uint64_t n, int rlog
 // merged samples are in smp[smi].  The split kernel, then the merge kernel.
 typedef hipError_t (*SegKwayLaunch)(const KSegH *segs, const uint2 *tmap, uint64_t ntiles, int pass, int rlog2,
                                     int slog2, int smi, uint32_t *err, hipStream_t s);
+// Every group of kw runs of 2^rlog2 items merged by one workgroup (no split
+// table; the padded group must fit one tile).
+typedef hipError_t (*KwayWholeLaunch)(const void *in, void *out, uint64_t n, int rlog2, uint32_t *err,
+                                      hipStream_t s);
 struct KwayShapeFns {
     int block, ipt;
     KwayLaunch run[5];  // fan-in 2, 4, 8, 16, 32
     SegKwayLaunch run_seg[5];
+    KwayWholeLaunch whole[5];
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
 // Writes tmap[tfirst[i] + k] = {list[i], k} for k < tfirst[i + 1] - tfirst[i].

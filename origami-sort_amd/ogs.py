@@ -308,7 +308,7 @@ class Context:
     TUNE_PARAMS = ("plain_block", "plain_merge", "plain_corank", "seg_tiny_block", "seg_tiny_max",
                    "seg_small_block", "seg_small_max", "seg_large_block", "seg_merge", "seg_plain_min",
                    "partition_lanes", "plain_fanin", "quad_lanes", "kway_fanin", "kway_shape", "kway_sample",
-                   "kway_scorank")
+                   "kway_scorank", "kway_swhole")
 
     def generate_ptr(self, kind, dist, seed, n, out, stream=0):
         """Counter-based input written on the device (ogs_generate)."""

@@ -314,3 +314,21 @@ def test_batched_kway_default_tiles(ctx, kind):
                 assert np.array_equal(g2.view(np.uint8), got.view(np.uint8))
     finally:
         plan.close()
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.F32, ogs.U64, ogs.I64, ogs.KV_U64U64, ogs.KV_U32U32])
+@pytest.mark.parametrize("swhole", [0, 1])
+def test_kway_sample_merge_forms(ctx, kind, swhole):
+    # a pass's sample merge as one K-way launch (a group's samples in one tile)
+    # or as 2-way merges / a block sort: the same sort either way
+    try:
+        ctx.tune_set(kind, "kway_swhole", swhole)
+        for n, dist in (((1 << 21) + 77, ogs.ZIPF), ((3 << 20) + 5, ogs.UNIFORM), (70001, ogs.DISTINCT16)):
+            say(f"case kway sample merge kind={ogs.KIND_NAMES[kind]} swhole={swhole} n={n}")
+            a = ogs.generate(kind, dist, n, seed=n + swhole)
+            if kind in (ogs.KV_U64U64, ogs.KV_U32U32):
+                check_kv(kind, gpu_sort(ctx, kind, a), a)
+            else:
+                check(kind, gpu_sort(ctx, kind, a), a)
+    finally:
+        ctx.tune_reset()
