@@ -774,6 +774,35 @@ static hipError_t launch_seg_kway(const KSegH *segs, const uint2 *tmap, uint64_t
     return hipGetLastError();
 }
 
+// The samples of every listed segment's groups of KW runs of 2^rlog2 sample
+// records (runs of the segment's sample array smp[0], whose length follows
+// from the segment's length and this pass's spacing) merged into smp[1], one
+// workgroup per group (tmap: {segment, group}); the padded group must fit one
+// tile, which the host checks.
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
+__global__ __launch_bounds__(BLOCK) void k_seg_kway_whole(const KSegH *__restrict__ segs,
+                                                          const uint2 *__restrict__ tmap, int pass, int rlog2,
+                                                          uint32_t *__restrict__ err) {
+    const uint2 m = tmap[blockIdx.x];
+    const KSegH &g = segs[m.x];
+    const int sl = (int)g.sl[pass];
+    const uint64_t ns = (g.len + (1ull << sl) - 1) >> sl;
+    const SampleSpec none = {nullptr, 0, 0, 0};
+    kway_merge_tile<K, HV, KW, BLOCK, IPT>(g.smp[0], g.smp[1], ns, rlog2, 1, 1, 0, nullptr, none, err, m.y);
+}
+
+template <typename K, bool HV, int KW, int BLOCK, int IPT>
+static hipError_t launch_seg_kway_whole(const KSegH *segs, const uint2 *tmap, uint64_t ngroups, int pass, int rlog2,
+                                        uint32_t *err, hipStream_t s) {
+    constexpr uint64_t C = (uint64_t)BLOCK * IPT;
+    if (ngroups == 0) return hipSuccess;
+    const uint64_t R = 1ull << rlog2;
+    if (rlog2 > 30 || (uint64_t)KW * ((R + IPT - 1) / IPT * IPT) > C) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_seg_kway_whole<K, HV, KW, BLOCK, IPT>), dim3((unsigned)ngroups), dim3(BLOCK), 0, s, segs,
+                       tmap, pass, rlog2, err);
+    return hipGetLastError();
+}
+
 #define OGS_KWAY_SHAPES_32(X) X(512, 32) X(256, 32) X(512, 16) X(256, 16) X(1024, 32) X(1024, 16)
 #define OGS_KWAY_SHAPES_64(X) X(256, 16) X(512, 16) X(256, 8) X(512, 8) X(1024, 16) X(1024, 8)
 #define OGS_KWAY_SHAPES_KV(X) X(512, 8) X(256, 16) X(256, 8) X(512, 4) X(1024, 4)
@@ -784,7 +813,10 @@ static hipError_t launch_seg_kway(const KSegH *segs, const uint2 *tmap, uint64_t
       &launch_seg_kway<K, HV, 16, B, I>, &launch_seg_kway<K, HV, 32, B, I>},                                \
      {&launch_kway_whole<K, HV, 2, B, I>, &launch_kway_whole<K, HV, 4, B, I>,                               \
       &launch_kway_whole<K, HV, 8, B, I>, &launch_kway_whole<K, HV, 16, B, I>,                              \
-      &launch_kway_whole<K, HV, 32, B, I>}},
+      &launch_kway_whole<K, HV, 32, B, I>},                                                                 \
+     {&launch_seg_kway_whole<K, HV, 2, B, I>, &launch_seg_kway_whole<K, HV, 4, B, I>,                       \
+      &launch_seg_kway_whole<K, HV, 8, B, I>, &launch_seg_kway_whole<K, HV, 16, B, I>,                      \
+      &launch_seg_kway_whole<K, HV, 32, B, I>}},
 
 static const KwayShapeFns kKway32[] = {
 #define X(B, I) OGS_KX(uint32_t, false, B, I)

@@ -702,18 +702,46 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
                     lvmax = std::max(lvmax, (int)k.lv[p]);
                     sl = k.sl[p];  // the same for every segment: it depends on rl alone
                 }
-            // group samples sorted by one u64 tile sort when a tile holds a group
-            // (the first pass: groups of 2^lv runs of 2^(rl - sl) samples), else
-            // lv levels of 2-way merges
+            // group samples merged by one K-way launch per fan-in when a group's
+            // samples fit one K-way tile, else sorted by one u64 tile sort when a
+            // tile holds a group (the first pass: groups of 2^lv runs of
+            // 2^(rl - sl) samples), else lv levels of 2-way merges
             const uint64_t span = 1ull << (rl - sl + lvmax);
+            const KwayShapeFns *kwh =
+                tune[ki.kc].v[TP_KWAY_SWHOLE] ? whole_group_shape(b.skc, 1 << lvmax, rl - sl) : nullptr;
             const BlockShapeFns *sbs = nullptr;
-            if (p == 0 && ki.kc == KC_32) {  // u64 samples are unique: a sort is the merge
+            if (!kwh && p == 0 && ki.kc == KC_32) {  // u64 samples are unique: a sort is the merge
                 int bcnt = 0;
                 const BlockShapeFns *bt = block_shapes(KC_64, &bcnt);
                 for (int j = 0; j < bcnt; j++)
                     if (bt[j].tile() >= span && (!sbs || bt[j].tile() < sbs->tile())) sbs = &bt[j];
             }
-            if (sbs) {
+            if (kwh) {
+                for (int lvv = 1; lvv <= lvmax; lvv++) {
+                    BatchBuild::KLaunch W;
+                    W.pass = p;
+                    W.lv = lvv;
+                    W.smi = 1;
+                    W.whole = kwh;
+                    W.tfirst.push_back(0);
+                    const uint64_t gs = 1ull << (rl - sl + lvv);  // sample records per group
+                    for (size_t i = 0; i < b.kseg.size(); i++) {
+                        const auto &k = b.kseg[i];
+                        if (k.passes <= p || k.lv[p] != lvv) continue;
+                        const uint64_t nsp = (b.recs[k.seg].len + (1ull << sl) - 1) >> sl;
+                        const uint64_t ng = (nsp + gs - 1) / gs;
+                        if (W.tfirst.back() + ng >= (1ull << 32)) return fail(OGS_ERR_ARG, "batch too large for one sample merge launch");
+                        W.list.push_back((uint32_t)i);
+                        W.tfirst.push_back(W.tfirst.back() + (uint32_t)ng);
+                    }
+                    if (W.list.empty()) continue;
+                    W.o_list = b.bytes;
+                    b.bytes = align(b.bytes + W.list.size() * sizeof(uint32_t));
+                    W.o_tfirst = b.bytes;
+                    b.bytes = align(b.bytes + W.tfirst.size() * sizeof(uint32_t));
+                    b.klaunch.push_back(std::move(W));
+                }
+            } else if (sbs) {
                 BatchBuild::SMerge m;
                 m.pass = p;
                 m.level = -1;
@@ -752,7 +780,7 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
                 BatchBuild::KLaunch L;
                 L.pass = p;
                 L.lv = lvv;
-                L.smi = sbs ? 1 : 1 + ((lvv - 1) & 1);
+                L.smi = kwh || sbs ? 1 : 1 + ((lvv - 1) & 1);
                 L.tfirst.push_back(0);
                 for (size_t i = 0; i < b.kseg.size(); i++) {
                     const auto &k = b.kseg[i];
@@ -901,7 +929,10 @@ int Ctx::batch_enqueue(const BatchBuild &b, void *D, uint64_t *part, hipStream_t
                 if (k.passes > p) sl = k.sl[p];
             hipError_t e = launch_kway_tilemap((const uint32_t *)(d + L.o_list), (const uint32_t *)(d + L.o_tfirst),
                                                (uint32_t)L.list.size(), (uint2 *)(d + L.o_tmap), s);
-            if (e == hipSuccess)
+            if (e == hipSuccess && L.whole)  // the pass's sample merge (listed before its K-way launches)
+                e = L.whole->whole_seg[L.lv - 1](kseg, (const uint2 *)(d + L.o_tmap), L.tfirst.back(), p, rl - sl,
+                                                 err, s);
+            else if (e == hipSuccess)
                 e = b.ks->run_seg[L.lv - 1](kseg, (const uint2 *)(d + L.o_tmap), L.tfirst.back(), p, rl, sl, L.smi,
                                             err, s);
             if (e != hipSuccess) return hip_fail(e, "batched K-way pass launch");

@@ -141,6 +141,7 @@ struct BatchBuild {
     std::vector<KSeg> kseg;
     struct KLaunch {             // pass p, fan-in 2^lv: its segments (indices into kseg)
         int pass, lv, smi;
+        const KwayShapeFns *whole = nullptr;  // set: the pass's sample merge, one workgroup per group
         std::vector<uint32_t> list, tfirst;
         size_t o_list = 0, o_tfirst = 0, o_tmap = 0;
     };

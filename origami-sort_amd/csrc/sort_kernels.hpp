@@ -174,11 +174,16 @@ typedef hipError_t (*SegKwayLaunch)(const KSegH *segs, const uint2 *tmap, uint64
 // table; the padded group must fit one tile).
 typedef hipError_t (*KwayWholeLaunch)(const void *in, void *out, uint64_t n, int rlog2, uint32_t *err,
                                       hipStream_t s);
+// Batched form: the samples of every listed segment's groups merged, one
+// workgroup per group (tmap: {segment, group}).
+typedef hipError_t (*SegKwayWholeLaunch)(const KSegH *segs, const uint2 *tmap, uint64_t ngroups, int pass,
+                                         int rlog2, uint32_t *err, hipStream_t s);
 struct KwayShapeFns {
     int block, ipt;
     KwayLaunch run[5];  // fan-in 2, 4, 8, 16, 32
     SegKwayLaunch run_seg[5];
     KwayWholeLaunch whole[5];
+    SegKwayWholeLaunch whole_seg[5];
     uint64_t tile() const { return (uint64_t)block * ipt; }
 };
 // Writes tmap[tfirst[i] + k] = {list[i], k} for k < tfirst[i + 1] - tfirst[i].

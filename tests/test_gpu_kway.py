@@ -332,3 +332,27 @@ def test_kway_sample_merge_forms(ctx, kind, swhole):
                 check(kind, gpu_sort(ctx, kind, a), a)
     finally:
         ctx.tune_reset()
+
+
+@pytest.mark.parametrize("kind", [ogs.U32, ogs.U64, ogs.KV_U64U64, ogs.KV_U32U32])
+@pytest.mark.parametrize("swhole", [0, 1])
+def test_batched_kway_sample_merge_forms(ctx, kind, swhole):
+    # batched segments with the samples of each pass merged by one K-way launch
+    # per fan-in (swhole 1) or by 2-way merges / a block sort (swhole 0)
+    import torch
+    rng = np.random.default_rng(40 + swhole)
+    lens = np.floor(2.0 ** (rng.random(120) * 18)).astype(np.int64)
+    lens = np.concatenate([lens, [1 << 18, (1 << 20) + 3, 2049, 65536]])
+    a, off, doff = batch_case(kind, ogs.ZIPF, lens, seed=kind + swhole)
+    n = a.size
+    say(f"case batched kway sample merge kind={ogs.KIND_NAMES[kind]} swhole={swhole} n={n}")
+    try:
+        ctx.tune_set(kind, "kway_swhole", swhole)
+        d = ogs.to_device(a)
+        s = torch.empty_like(d)
+        p = ctx.segmented_ptr(kind, d.data_ptr(), s.data_ptr(), doff.data_ptr(), lens.size, n)
+        torch.cuda.synchronize()
+        got = ogs.from_device(d if p == d.data_ptr() else s, kind, n)
+    finally:
+        ctx.tune_reset()
+    check_segments(kind, got, a, off)
