@@ -303,11 +303,16 @@ __device__ __forceinline__ void store_chunks(const K *sk, K *ob, int cnt, int ti
     const CodecMask<K> m = codec_mask<K>(CODEC);
     auto put = [&](int k, const K *x) {
         K *d = ob + (uint32_t)(E * (tid + k * BLOCK));
+        // non-temporal: the pass's output is next read by the following pass,
+        // after far more data than the caches hold
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
         if constexpr (E == 4)
-            *(uint4 *)d = uint4{(uint32_t)dec_bf(x[0], m), (uint32_t)dec_bf(x[1], m), (uint32_t)dec_bf(x[2], m),
-                                (uint32_t)dec_bf(x[3], m)};
+            __builtin_nontemporal_store(u32x4{(uint32_t)dec_bf(x[0], m), (uint32_t)dec_bf(x[1], m),
+                                              (uint32_t)dec_bf(x[2], m), (uint32_t)dec_bf(x[3], m)},
+                                        (u32x4 *)d);
         else
-            *(ulonglong2 *)d = ulonglong2{(unsigned long long)dec_bf(x[0], m), (unsigned long long)dec_bf(x[1], m)};
+            __builtin_nontemporal_store(u64x2{(uint64_t)dec_bf(x[0], m), (uint64_t)dec_bf(x[1], m)}, (u64x2 *)d);
     };
     K v[IPT];
     if (tid + (NK - 1) * BLOCK < nb) {

@@ -650,22 +650,26 @@ __device__ __forceinline__ void lds_store_tile(const K *sk, const uint64_t *sv, 
 #pragma unroll
             for (int q = tid; q < TILE / E; q += BLOCK) {
                 const uint32_t e = (uint32_t)(q * E);
+                // non-temporal: a pass's output is next read after far more
+                // data than the caches hold
+                typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
                 if constexpr (E == 4) {
-                    uint4 x = *(const uint4 *)&sk[lsw<K>(e)];
+                    u32x4 x = *(const u32x4 *)&sk[lsw<K>(e)];
                     if (cout) {
                         x.x = dec_bf<uint32_t>(x.x, m);
                         x.y = dec_bf<uint32_t>(x.y, m);
                         x.z = dec_bf<uint32_t>(x.z, m);
                         x.w = dec_bf<uint32_t>(x.w, m);
                     }
-                    *(uint4 *)&o[e] = x;
+                    __builtin_nontemporal_store(x, (u32x4 *)&o[e]);
                 } else {
-                    ulonglong2 x = *(const ulonglong2 *)&sk[lsw<K>(e)];
+                    u64x2 x = *(const u64x2 *)&sk[lsw<K>(e)];
                     if (cout) {
-                        x.x = dec_bf<uint64_t>((uint64_t)x.x, m);
-                        x.y = dec_bf<uint64_t>((uint64_t)x.y, m);
+                        x.x = dec_bf<uint64_t>(x.x, m);
+                        x.y = dec_bf<uint64_t>(x.y, m);
                     }
-                    *(ulonglong2 *)&o[e] = x;
+                    __builtin_nontemporal_store(x, (u64x2 *)&o[e]);
                 }
             }
         } else {
