@@ -109,24 +109,39 @@ struct GMem {
     }
 };
 
-// LDS layout: XOR swizzle of the 16-byte chunk index within each 128-byte
-// row by the row number.  A lane writing its IPT consecutive items as 16-byte
-// stores then hits distinct banks from its neighbours (plain layout: lanes 0
-// and 2 collide), and lane-consecutive accesses stay a permutation within a
-// row.  Items of width T; the swizzle never moves an item out of its chunk.
+// LDS layout: an XOR swizzle of the 16-byte chunk bits of the byte offset by
+// higher bits.  A lane writing its IPT consecutive items as 16-byte stores
+// then hits distinct banks from its neighbours (plain layout: lanes 0 and 4
+// collide), and lane-consecutive accesses stay a permutation of the banks.
+//   4-byte items: byte bits 4..7 ^= byte bits 6..9 (item bits 2..5 ^= 4..7);
+//   8-byte items: byte bits 4..6 ^= byte bits 7..9 (item bits 1..3 ^= 4..6).
+// Either way the byte offset of item i is (i * W) ^ (i & M) -- a shift and one
+// v_bitop3 from the item index, which merge loops keep as their pointers --
+// the map is XOR-linear (so an aligned block of items is its first item's
+// offset XOR constants), it repeats every 256 (4-byte) or 128 (8-byte) items,
+// and it never moves an item out of its 16-byte chunk.
 template <typename T> __device__ __forceinline__ int lsw(int i) {
-    if constexpr (sizeof(T) == 4) return i ^ (((i >> 5) & 7) << 2);
+    if constexpr (sizeof(T) == 4) return i ^ ((i >> 2) & 0x3C);
     else return i ^ (((i >> 4) & 7) << 1);
 }
 // items per 16-byte chunk
 template <typename T> constexpr int chunk_items() { return 16 / (int)sizeof(T); }
 
-// The swizzle on byte offsets (for 4- and 8-byte items alike: the 16-byte
-// chunk index within a 128-byte row XOR the row number): lets merge loops keep
-// their pointers in bytes and address LDS with two instructions.
-__device__ __forceinline__ int lsw_bytes(int b) { return b ^ ((b >> 3) & 0x70); }
+// Swizzled byte offset of LDS item i (= lsw<K>(i) * sizeof(K)).
+template <typename K> __device__ __forceinline__ int lds_off(int i) {
+    if constexpr (sizeof(K) == 4) return (i << 2) ^ (i & 0xF0);
+    else return (i << 3) ^ (i & 0x70);
+}
+// The swizzle on byte offsets b = i * sizeof(K).
+template <typename K> __device__ __forceinline__ int lsw_bytes(int b) {
+    if constexpr (sizeof(K) == 4) return b ^ ((b >> 2) & 0xF0);
+    else return b ^ ((b >> 3) & 0x70);
+}
 template <typename K> __device__ __forceinline__ K lds_at_bytes(const K *sk, int b) {
-    return *(const K *)((const char *)sk + lsw_bytes(b));
+    return *(const K *)((const char *)sk + lsw_bytes<K>(b));
+}
+template <typename K> __device__ __forceinline__ K lds_at(const K *sk, int i) {
+    return *(const K *)((const char *)sk + lds_off<K>(i));
 }
 
 // ----------------------------------------------------------------------------
@@ -196,7 +211,17 @@ template <int M> __device__ __forceinline__ uint64_t xshfl(uint64_t x) {
 // own item, so no item is duplicated or lost.
 template <bool HV, typename K>
 __device__ __forceinline__ void lane_pair_keep(K &mine, uint64_t &vm, K other, uint64_t vo, bool upper) {
-    if constexpr (!HV) {
+    if constexpr (!HV && sizeof(K) == 4) {
+        // key-only 32-bit: the median of (mine, other, 0 or ~0) is the min
+        // for the lower lane and the max for the upper one -- one v_med3_u32
+        // (the partner's item is one DPP move or swizzle before it)
+        const K c = upper ? KeyMax<K>::v : (K)0;
+        const K lo = other < mine ? other : mine;
+        const K hi = other < mine ? mine : other;
+        const K t = hi < c ? hi : c;
+        mine = lo < t ? t : lo;
+        (void)vm; (void)vo;
+    } else if constexpr (!HV) {
         // key-only: equal keys are identical bits, so the pair is min / max
         const K lo = other < mine ? other : mine;
         const K hi = other < mine ? mine : other;
@@ -429,32 +454,29 @@ template <int IPT, bool FULL, typename K>
 __device__ __forceinline__ void lds_merge_rev(const K *sk, int ia, int ib, int i0, int cnt, K (&r)[IPT]) {
     K ka = sk[lsw<K>(ia)], kb = sk[lsw<K>(ib)];
     if constexpr (FULL) {
-        // One pointer: after i + 1 outputs the B pointer is ib0 - (i + 1 - (ia -
-        // ia0)) = ia + dlt - (i + 1), with dlt = ib0 - ia0, so each step updates
-        // ia alone and picks the next LDS index as ia or ia + dlt - (i + 1).
-        // pointers in bytes (W per item)
-        constexpr int W = (int)sizeof(K);
-        const int dlt = (ib - ia) * W;
-        int pa = ia * W;
         // Key-only, so equal keys are identical bits and ties may go either
         // way: the registers hold the head that lost the last compare (L) and
         // the item read last (N, from run A when nA), the output is min(N, L),
         // the loser max(N, L), and the winner's run is a lane mask (scalar
-        // instructions).  Saves the two head selects of the form below.
+        // instructions).  The pointers are LDS item indices: the winner's
+        // pointer moves by the mask itself (one add with carry-in each:
+        // ia + fromA, ib - 1 + fromA), and the next read's byte offset is a
+        // shift and a v_bitop3 (lds_off): eight VALU instructions per output.
         K N = ka, L = kb;
         bool nA = true;
+        const int dlt = ib - ia;
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
-            const bool wN = !(L < N);  // N wins (ties: either)
-            const K lo = wN ? N : L;
-            L = wN ? L : N;
+            const bool lN = L < N;  // L wins (ties: N)
+            const K lo = lN ? L : N;
+            L = lN ? N : L;
             r[i] = lo;
-            const bool fromA = wN == nA;  // the winner's run
-            const int pa1 = pa + W;
-            const int nb = pa + dlt - W * (i + 1);
-            const int nx = fromA ? pa1 : nb;
-            pa = fromA ? pa1 : pa;
-            N = lds_at_bytes(sk, nx);
+            const bool fromA = lN != nA;  // the winner's run
+            ia += (int)fromA;
+            asm("" : "+v"(ia));
+            int nb = ia + dlt - (i + 1);
+            asm("" : "+v"(nb));
+            N = lds_at(sk, fromA ? ia : nb);
             nA = fromA;
         }
         return;
@@ -512,17 +534,42 @@ __device__ __forceinline__ int lds_corank_split_rev(const K *sk, const K *spF, c
         const K kb = *(const K *)(pB - tq);
         abq = kb < ka ? abq : tq;
     }
-    // the last steps on the items (in bytes), from the block-granular answer
+    // the last steps on the items (in bytes), from the block-granular answer;
+    // steps of WS = min(IPT, 16) items or more on any offsets
+    constexpr int WS = IPT < 16 ? IPT : 16;
     const int pa = (a_off - 1) * (int)W, pb = (b_top - diag) * (int)W, hib = (int)(hi * W);
     int ab = (int)(abq * (uint32_t)IPT);
 #pragma unroll
-    for (int s = IPT / 2 * (int)W; s >= (int)W; s >>= 1) {
+    for (int s = IPT / 2 * (int)W; s >= WS * (int)W; s >>= 1) {
         const int tb = min(ab + s, hib);
         const K ka = lds_at_bytes(sk, pa + tb);
         const K kb = lds_at_bytes(sk, pb + tb);
         ab = kb < ka ? ab : tb;
     }
-    return (int)((uint32_t)ab / W);
+    // Steps below WS items stay inside two WS-aligned item windows: with a0 =
+    // the answer so far (a multiple of WS), the probe a0 + v + 1 compares A
+    // item xa + v with B item yb + v (xa = a_off + a0, yb = b_top - diag + a0
+    // + 1, both multiples of WS, as every run offset, length and diagonal here
+    // is a multiple of IPT); for x a multiple of WS and v < WS <= 16, x + v =
+    // x ^ v and the swizzle is XOR-linear with lds_off(v) = v * W, so item
+    // x + v sits at lds_off(x) ^ (v * W): each step is an add, a min, two
+    // v_bitop3 (offset XOR (v*W masked to the window)), a compare and a
+    // select.  v is kept as v * W from -W (no A item past a0 taken); a probe
+    // clamped to -W reads inside the window (the mask) and cannot move.
+    const int a0 = (int)((uint32_t)ab / W);
+    const int swa = lds_off<K>(a_off + a0), swb = lds_off<K>(b_top - diag + a0 + 1);
+    const int hv = (hi - (int)a0 - 1) * (int)W;  // largest probe v * W
+    constexpr int MW = (WS - 1) * (int)W;
+    const char *sb = (const char *)sk;
+    int vb = -(int)W;
+#pragma unroll
+    for (int s = WS / 2 * (int)W; s >= (int)W; s >>= 1) {
+        const int tb = min(vb + s, hv);
+        const K ka = *(const K *)(sb + (swa ^ (tb & MW)));
+        const K kb = *(const K *)(sb + (swb ^ (tb & MW)));
+        vb = kb < ka ? vb : tb;
+    }
+    return a0 + vb / (int)W + 1;
 }
 
 // Largest power of two <= x (0 for 0): the co-rank step bound for runs whose
@@ -540,30 +587,35 @@ __device__ __forceinline__ void lds_merge_slice(const K *sk, const uint64_t *sv,
 }
 
 // Store IPT consecutive registers to LDS at item index base (a multiple of
-// the chunk size) as 16-byte stores.
+// IPT, a power of two) as 16-byte stores: chunk c of the block is at
+// lds_off(base) ^ lds_off(c), the swizzle being XOR-linear and base + c = base
+// ^ c, so one address per thread and a constant XOR per chunk.
 template <int IPT, bool HV, typename K>
 __device__ __forceinline__ void lds_put(K *sk, uint64_t *sv, int base, const K (&r)[IPT], const uint64_t (&v)[IPT]) {
     constexpr int E = chunk_items<K>();
-    static_assert(IPT % E == 0, "IPT must fill whole 16-byte chunks");
+    static_assert(IPT % E == 0 && (IPT & (IPT - 1)) == 0, "IPT: a power of two of whole 16-byte chunks");
+    const int sw = lds_off<K>(base);
 #pragma unroll
     for (int c = 0; c < IPT; c += E) {
+        char *p = (char *)sk + (sw ^ lds_off<K>(c));
         if constexpr (E == 4) {
             uint4 q = {(uint32_t)r[c], (uint32_t)r[c + 1], (uint32_t)r[c + 2], (uint32_t)r[c + 3]};
-            *(uint4 *)&sk[lsw<K>(base + c)] = q;
+            *(uint4 *)p = q;
         } else {
             ulonglong2 q;
             q.x = (unsigned long long)r[c];
             q.y = (unsigned long long)r[c + 1];
-            *(ulonglong2 *)&sk[lsw<K>(base + c)] = q;
+            *(ulonglong2 *)p = q;
         }
     }
     if constexpr (HV) {
+        const int swv = lds_off<uint64_t>(base);
 #pragma unroll
         for (int c = 0; c < IPT; c += 2) {
             ulonglong2 q;
             q.x = v[c];
             q.y = v[c + 1];
-            *(ulonglong2 *)&sv[lsw<uint64_t>(base + c)] = q;
+            *(ulonglong2 *)((char *)sv + (swv ^ lds_off<uint64_t>(c))) = q;
         }
     }
 }
@@ -571,44 +623,25 @@ __device__ __forceinline__ void lds_put(K *sk, uint64_t *sv, int base, const K (
 // Stores IPT registers, items [o, o + IPT) of a run that the next merge reads
 // descending, at LDS indices top, top - 1, ..., top - IPT + 1 (top + 1 a
 // multiple of IPT, which every caller's block alignment gives) as 16-byte
-// stores.
+// stores: the block starts at base = top + 1 - IPT, and the chunk of LDS items
+// base + x .. base + x + E - 1 (x = IPT - E - c) is at lds_off(base) ^
+// lds_off(x) (XOR-linear swizzle, base + x = base ^ x).
 template <int IPT, typename K>
 __device__ __forceinline__ void lds_put_desc(K *sk, int top, const K (&r)[IPT]) {
     constexpr int E = chunk_items<K>();
-    static_assert(IPT % E == 0, "IPT must fill whole 16-byte chunks");
-    if constexpr (IPT * sizeof(K) <= 128) {
-        // top + 1 is a multiple of IPT here, so the IPT items are one aligned
-        // block inside a 128-byte row: every chunk shares the row's swizzle
-        // term, and chunk k of the block (from its low end) is at
-        // (base ^ term) ^ 16 k
-        const int base = (top + 1 - IPT) * (int)sizeof(K);
-        const int sw = base ^ ((base >> 3) & 0x70);
-#pragma unroll
-        for (int c = 0; c < IPT; c += E) {
-            const int addr = sw ^ ((IPT - E - c) * (int)sizeof(K));
-            if constexpr (E == 4) {
-                uint4 q = {(uint32_t)r[c + 3], (uint32_t)r[c + 2], (uint32_t)r[c + 1], (uint32_t)r[c]};
-                *(uint4 *)((char *)sk + addr) = q;
-            } else {
-                ulonglong2 q;
-                q.x = (unsigned long long)r[c + 1];
-                q.y = (unsigned long long)r[c];
-                *(ulonglong2 *)((char *)sk + addr) = q;
-            }
-        }
-        return;
-    }
+    static_assert(IPT % E == 0 && (IPT & (IPT - 1)) == 0, "IPT: a power of two of whole 16-byte chunks");
+    const int sw = lds_off<K>(top + 1 - IPT);
 #pragma unroll
     for (int c = 0; c < IPT; c += E) {
-        const int lo = top - c - (E - 1);
+        char *p = (char *)sk + (sw ^ lds_off<K>(IPT - E - c));
         if constexpr (E == 4) {
             uint4 q = {(uint32_t)r[c + 3], (uint32_t)r[c + 2], (uint32_t)r[c + 1], (uint32_t)r[c]};
-            *(uint4 *)&sk[lsw<K>(lo)] = q;
+            *(uint4 *)p = q;
         } else {
             ulonglong2 q;
             q.x = (unsigned long long)r[c + 1];
             q.y = (unsigned long long)r[c];
-            *(ulonglong2 *)&sk[lsw<K>(lo)] = q;
+            *(ulonglong2 *)p = q;
         }
     }
 }
