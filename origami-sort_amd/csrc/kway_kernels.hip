@@ -375,8 +375,11 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
     // run's ascending order), for the block-granular co-rank steps; one guard
     // entry either side (a step clamped to its current answer reads index -1
     // or BLOCK, and its result does not depend on the value)
-    __shared__ K s_spFb[HV ? 1 : BLOCK + 2], s_spLb[HV ? 1 : BLOCK + 2];
-    K *s_spF = s_spFb + (HV ? 0 : 1), *s_spL = s_spLb + (HV ? 0 : 1);
+    // The same LDS first holds, during the load phase, each chunk's source
+    // (8 bytes per thread: the narrow form's cb_lo and cmeta below).
+    __shared__ __attribute__((aligned(16))) K s_spb[HV ? 2 : 2 * (BLOCK + 2)];
+    K *s_spF = s_spb + (HV ? 0 : 1), *s_spL = s_spb + (HV ? 1 : BLOCK + 3);
+    static_assert(HV || 2 * (BLOCK + 2) * sizeof(K) >= 8 * BLOCK, "chunk sources fit the splitter arrays");
     K *sk = skb + lds_front<K>();
     uint64_t *sv = svb + (HV ? lds_front<K>() : 0);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -511,7 +514,30 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
         const int wbase = (tid >> 6) * 64 * IPT;
         K kk[IPT];
         uint64_t vv[IPT];
-        if (narrow) {
+        if (narrow && !HV) {
+            // key-only: every thread leaves its chunk's (cb_lo, cmeta) in LDS
+            // and the lanes of each load read their chunk's pair back (one
+            // 8-byte read that 64 / IPT lanes share, instead of three
+            // ds_bpermute); a descending chunk's cb_lo = B + 1 is odd (B is a
+            // multiple of the item size), so d is its low bit.  Each wave
+            // reads only its own threads' pairs, and a wave's LDS accesses
+            // complete in order.
+            uint2 *s_src2 = (uint2 *)s_spb;
+            s_src2[tid] = make_uint2(cb_lo, cmeta);
+            const uint2 *sq = s_src2 + (tid & ~63) + lane / IPT;
+            const char *gb = (const char *)in + gbase * ES;  // uniform: scalar registers
+            const uint32_t rb = (uint32_t)r * ES;
+#pragma unroll
+            for (int i = 0; i < IPT; i++) {
+                // position within the wave's range: BLK * (i % NB) + 64 * (i / NB) + lane
+                const uint2 src = sq[(BLK / IPT) * (i % NB) + (64 / IPT) * (i / NB)];
+                const uint32_t bb = src.x, pm = src.y;
+                const uint32_t d = (uint32_t)__builtin_amdgcn_sbfe((int)bb, 0u, 1u);
+                const int p = __builtin_amdgcn_sbfe((int)pm, (uint32_t)r, 1u);
+                const uint32_t off = ((rb ^ d) + bb) & ~(uint32_t)p;
+                kk[i] = *(const K *)(gb + off) | (K)(typename std::make_signed<K>::type)p;
+            }
+        } else if (narrow) {
             const char *gb = (const char *)in + gbase * ES;  // uniform: scalar registers
             const uint32_t rb = (uint32_t)r * ES;
 #pragma unroll
