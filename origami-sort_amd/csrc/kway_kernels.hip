@@ -371,15 +371,20 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
     __shared__ int s_cnt;           // items in the tile (-1: inconsistent table, reported in *err)
     __shared__ int s_top[LOGKW + 1];  // per level: co-rank step bound over all pairs
     // key-only: the first and last item of every IPT-item block of the current
-    // runs (block k of the run at LDS offset o at index o / IPT + k, in the
+    // runs (block k of the run at LDS offset o as pair o / IPT + k, in the
     // run's ascending order), for the block-granular co-rank steps; one guard
-    // entry either side (a step clamped to its current answer reads index -1
-    // or BLOCK, and its result does not depend on the value)
+    // pair either side (a step clamped to its current answer reads pair -1 or
+    // BLOCK, and its result does not depend on the value).
     // The same LDS first holds, during the load phase, each chunk's source
     // (8 bytes per thread: the narrow form's cb_lo and cmeta below).
     __shared__ __attribute__((aligned(16))) K s_spb[HV ? 2 : 2 * (BLOCK + 2)];
-    K *s_spF = s_spb + (HV ? 0 : 1), *s_spL = s_spb + (HV ? 1 : BLOCK + 3);
-    static_assert(HV || 2 * (BLOCK + 2) * sizeof(K) >= 8 * BLOCK, "chunk sources fit the splitter arrays");
+    K *s_spFL = s_spb + (HV ? 0 : 2);
+    static_assert(HV || 2 * (BLOCK + 2) * sizeof(K) >= 8 * BLOCK, "chunk sources fit the splitter pairs");
+    // key-only, per level lev and pair p of the level (slices [p * 2^lev,
+    // (p + 1) * 2^lev)), at entry KW - 2 * (KW >> lev) + p: the pair's LDS
+    // offset, its second run's offset, its end and the level's co-rank step
+    // bound (the largest over the level's pairs) -- one 16-byte read per level
+    __shared__ int4 s_pair[HV ? 1 : KW];
     K *sk = skb + lds_front<K>();
     uint64_t *sv = svb + (HV ? lds_front<K>() : 0);
     const int tid = threadIdx.x, lane = tid & 63;
@@ -422,7 +427,29 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
         }
     }
     __syncthreads();
-    if (tid >= 1 && tid <= LOGKW) {
+    if (!HV && tid < 64) {
+        // wave 0: lane e < KW - 1 builds entry e (level lev, pair p); the
+        // level's bound is a max over its pairs, which are an aligned group of
+        // KW >> lev lanes: a butterfly within the group
+        const int e = tid;
+        const int u = KW - 1 - e;
+        const int lev = u >= 1 ? LOGKW - (31 - __clz(u)) : LOGKW + 1;  // lanes e >= KW - 1: none
+        int a_off = 0, mid = 0, end = 0, mx = 0;
+        if (lev <= LOGKW) {
+            const int hv = 1 << (lev - 1), p = e - (KW - 2 * (KW >> lev));
+            a_off = s_P[p * 2 * hv];
+            mid = s_P[p * 2 * hv + hv];
+            end = s_P[p * 2 * hv + 2 * hv];
+            mx = min(mid - a_off, end - mid);
+        }
+        const int gsz = lev <= LOGKW ? KW >> lev : 1;
+        if constexpr (KW >= 4) { const int o = (int)xshfl<1>((uint32_t)mx); mx = gsz > 1 ? max(mx, o) : mx; }
+        if constexpr (KW >= 8) { const int o = (int)xshfl<2>((uint32_t)mx); mx = gsz > 2 ? max(mx, o) : mx; }
+        if constexpr (KW >= 16) { const int o = (int)xshfl<4>((uint32_t)mx); mx = gsz > 4 ? max(mx, o) : mx; }
+        if constexpr (KW >= 32) { const int o = (int)xshfl<8>((uint32_t)mx); mx = gsz > 8 ? max(mx, o) : mx; }
+        if (lev <= LOGKW) s_pair[e] = make_int4(a_off, mid, end, pow2_floor(mx));
+    }
+    if (HV && tid >= 1 && tid <= LOGKW) {
         // the co-rank of a pair (la, lb) needs pow2_floor(min(la, lb)) lifting
         // steps (KV: the real lengths); every thread of a level takes the
         // largest, so the step loop is uniform (more steps than a pair needs
@@ -598,11 +625,9 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
             const K first = sk[lsw<K>(c0)], last = sk[lsw<K>(c0 + IPT - 1)];
             if (j0 & 1) {
                 const int idx = (s_P[j0] + s_P[j0 + 1] - c0) / IPT - 1;
-                s_spF[idx] = last;
-                s_spL[idx] = first;
+                put_pair(s_spFL + 2 * idx, last, first);
             } else {
-                s_spF[tid] = first;
-                s_spL[tid] = last;
+                put_pair(s_spFL + 2 * tid, first, last);
             }
         }
         __syncthreads();
@@ -613,17 +638,22 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
     for (int lev = 1; lev <= LOGKW; lev++) {
         int end = 0, diag = 0;
         if (live) {
-            const int hv = 1 << (lev - 1);
-            const int s0 = (j0 >> lev) << lev;
-            const int a_off = s_P[s0], mid = s_P[s0 + hv];
-            end = s_P[s0 + 2 * hv];
-            diag = c0 - a_off;
-            const int top = __builtin_amdgcn_readfirstlane(s_top[lev]);
             if constexpr (!HV) {
-                const int a = lds_corank_split_rev<IPT>(sk, s_spF, s_spL, a_off, mid - a_off, mid, end - mid, end - 1,
-                                                        diag, top);
+                const int4 pr = s_pair[KW - 2 * (KW >> lev) + (j0 >> lev)];
+                const int a_off = pr.x, mid = pr.y;
+                end = pr.z;
+                diag = c0 - a_off;
+                const int top = __builtin_amdgcn_readfirstlane(pr.w);
+                const int a = lds_corank_split_rev<IPT>(sk, s_spFL, a_off, mid - a_off, mid, end - mid, end - 1, diag,
+                                                        top);
                 lds_merge_rev<IPT, true>(sk, a_off + a, end - 1 - (diag - a), 0, IPT, r);
             } else {
+                const int hv = 1 << (lev - 1);
+                const int s0 = (j0 >> lev) << lev;
+                const int a_off = s_P[s0], mid = s_P[s0 + hv];
+                end = s_P[s0 + 2 * hv];
+                diag = c0 - a_off;
+                const int top = __builtin_amdgcn_readfirstlane(s_top[lev]);
                 // KV: ascending runs; merges test exhaustion against the real
                 // lengths, so a pad never displaces a real maximum-key item and
                 // the merged pair is its real items, then pads
@@ -651,10 +681,7 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
             if constexpr (!HV) {
                 // this thread's outputs are block (c0 - pair start) / IPT of
                 // the merged run at the pair's offset: splitter index tid
-                if (lev < LOGKW) {
-                    s_spF[tid] = r[0];
-                    s_spL[tid] = r[IPT - 1];
-                }
+                if (lev < LOGKW) put_pair(s_spFL + 2 * tid, r[0], r[IPT - 1]);
             }
         }
         __syncthreads();

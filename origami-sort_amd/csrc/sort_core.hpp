@@ -506,29 +506,33 @@ __device__ __forceinline__ void lds_merge_slice_rev(const K *sk, int a_off, int 
 
 // Co-rank on the bitonic layout with block splitters: runs whose lengths and
 // offsets are multiples of IPT, a diagonal d that is one too, and for every
-// IPT-item block of a run (block k of the run at LDS offset o: splitter index
+// IPT-item block of a run (block k of the run at LDS offset o: splitter pair
 // o / IPT + k, in the run's ascending order whatever its storage direction)
-// its first item in spF and its last in spL.  The lifting steps of IPT items
-// and more compare splitters -- A[t-1] is the last item of A's block t/IPT - 1,
-// B[d-t] the first of B's block (d-t)/IPT -- on small unswizzled arrays, six
-// instructions per step instead of ten; the last log2(IPT) steps run on the
-// items themselves.  The same answer as lds_corank_lift_rev for any top with
-// 2 * top > min(la, lb).
+// its first item at spFL[2 * pair] and its last at spFL[2 * pair + 1] (one
+// 2-item store per block).  The lifting steps of IPT items and more compare
+// splitters -- A[t-1] is the last item of A's block t/IPT - 1, B[d-t] the first
+// of B's block (d-t)/IPT -- on a small unswizzled array, six instructions per
+// step instead of ten; the last log2(IPT) steps run on the items themselves.
+// The same answer as lds_corank_lift_rev for any top with 2 * top > min(la, lb).
 template <int IPT, typename K>
-__device__ __forceinline__ int lds_corank_split_rev(const K *sk, const K *spF, const K *spL, int a_off, int la,
-                                                    int b_off, int lb, int b_top, int diag, int top) {
+__device__ __forceinline__ int lds_corank_split_rev(const K *sk, const K *spFL, int a_off, int la, int b_off, int lb,
+                                                    int b_top, int diag, int top) {
     static_assert((IPT & (IPT - 1)) == 0 && IPT >= (int)sizeof(K), "IPT: a power of two, at least the key bytes");
-    constexpr uint32_t W = (uint32_t)sizeof(K), SH = (uint32_t)__builtin_ctz((unsigned)IPT / W);
+    constexpr uint32_t W = (uint32_t)sizeof(K);
     // every length and offset here is a non-negative multiple of IPT: an
-    // item count x is x >> SH splitter bytes
+    // item count x is x * 2W / IPT bytes of splitter pairs
+    constexpr bool DN = IPT >= 2 * (int)W;
+    constexpr uint32_t SH = DN ? (uint32_t)__builtin_ctz((unsigned)IPT / (2 * W))
+                               : (uint32_t)__builtin_ctz(2 * W / (unsigned)IPT);
+    const auto cv = [](uint32_t x) -> uint32_t { return DN ? x >> SH : x << SH; };
     const uint32_t lo = (uint32_t)(diag > lb ? diag - lb : 0);
     const uint32_t hi = (uint32_t)(diag < la ? diag : la);
-    // block-granular steps: A[t-1] at pA + (t >> SH), B[diag-t] at pB - (t >> SH)
-    const char *pA = (const char *)spL + ((uint32_t)a_off >> SH) - W;
-    const char *pB = (const char *)spF + ((uint32_t)(b_off + diag) >> SH);
-    const uint32_t hiq = hi >> SH;
-    uint32_t abq = lo >> SH;
-    for (uint32_t s = (uint32_t)top >> SH; s >= W; s >>= 1) {
+    // block-granular steps: A[t-1] at pA + cv(t), B[diag-t] at pB - cv(t)
+    const char *pA = (const char *)spFL + cv((uint32_t)a_off) - W;
+    const char *pB = (const char *)spFL + cv((uint32_t)(b_off + diag));
+    const uint32_t hiq = cv(hi);
+    uint32_t abq = cv(lo);
+    for (uint32_t s = cv((uint32_t)top); s >= 2 * W; s >>= 1) {
         const uint32_t tq = min(abq + s, hiq);
         const K ka = *(const K *)(pA + tq);
         const K kb = *(const K *)(pB - tq);
@@ -538,7 +542,7 @@ __device__ __forceinline__ int lds_corank_split_rev(const K *sk, const K *spF, c
     // steps of WS = min(IPT, 16) items or more on any offsets
     constexpr int WS = IPT < 16 ? IPT : 16;
     const int pa = (a_off - 1) * (int)W, pb = (b_top - diag) * (int)W, hib = (int)(hi * W);
-    int ab = (int)(abq * (uint32_t)IPT);
+    int ab = (int)((DN ? abq << SH : abq >> SH) * W);
 #pragma unroll
     for (int s = IPT / 2 * (int)W; s >= WS * (int)W; s >>= 1) {
         const int tb = min(ab + s, hib);
@@ -570,6 +574,12 @@ __device__ __forceinline__ int lds_corank_split_rev(const K *sk, const K *spF, c
         vb = kb < ka ? vb : tb;
     }
     return a0 + vb / (int)W + 1;
+}
+
+// A block's splitter pair (its first and last key) as one 8- or 16-byte store.
+template <typename K> __device__ __forceinline__ void put_pair(K *p, K first, K last) {
+    if constexpr (sizeof(K) == 4) *(uint2 *)p = make_uint2(first, last);
+    else *(ulonglong2 *)p = make_ulonglong2((unsigned long long)first, (unsigned long long)last);
 }
 
 // Largest power of two <= x (0 for 0): the co-rank step bound for runs whose

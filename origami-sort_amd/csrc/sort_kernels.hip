@@ -108,10 +108,10 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
     lane_network<IPT, HV>(r, v);
     wave_sort_levels<WLEV, IPT, HV>(r, v, lane);
     // key-only: the first and last key of every IPT-item block of the current
-    // runs (thread tid's block, in its run's ascending order, at index tid),
-    // for the co-rank's steps of a block or more; a guard entry either side
-    __shared__ K s_spFb[HV ? 1 : BLOCK + 2], s_spLb[HV ? 1 : BLOCK + 2];
-    K *s_spF = s_spFb + (HV ? 0 : 1), *s_spL = s_spLb + (HV ? 0 : 1);
+    // runs (thread tid's block, in its run's ascending order, as pair tid),
+    // for the co-rank's steps of a block or more; a guard pair either side
+    __shared__ __attribute__((aligned(16))) K s_spb[HV ? 2 : 2 * (BLOCK + 2)];
+    K *s_spFL = s_spb + (HV ? 0 : 2);
 #pragma unroll 1
     for (int R = IPT << WLEV; R < TILE; R <<= 1) {
         const int o = tid * IPT;
@@ -122,10 +122,9 @@ __device__ __forceinline__ void tile_sort(const void *in, void *out, uint64_t ba
             // odd runs of length R are the B side of this round: descending
             if (o & R) lds_put_desc<IPT>(sk, 2 * (o & ~(R - 1)) + R - 1 - o, r);
             else lds_put<IPT, HV>(sk, sv, o, r, v);
-            s_spF[tid] = r[0];
-            s_spL[tid] = r[IPT - 1];
+            put_pair(s_spFL + 2 * tid, r[0], r[IPT - 1]);
             __syncthreads();
-            const int a = lds_corank_split_rev<IPT>(sk, s_spF, s_spL, pb, R, pb + R, R, pb + 2 * R - 1, diag, R);
+            const int a = lds_corank_split_rev<IPT>(sk, s_spFL, pb, R, pb + R, R, pb + 2 * R - 1, diag, R);
             lds_merge_rev<IPT, true>(sk, pb + a, pb + 2 * R - 1 - (diag - a), 0, IPT, r);
         } else {
             lds_put<IPT, HV>(sk, sv, o, r, v);
