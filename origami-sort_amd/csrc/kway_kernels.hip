@@ -375,11 +375,8 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
     // run's ascending order), for the block-granular co-rank steps; one guard
     // pair either side (a step clamped to its current answer reads pair -1 or
     // BLOCK, and its result does not depend on the value).
-    // The same LDS first holds, during the load phase, each chunk's source
-    // (8 bytes per thread: the narrow form's cb_lo and cmeta below).
     __shared__ __attribute__((aligned(16))) K s_spb[HV ? 2 : 2 * (BLOCK + 2)];
     K *s_spFL = s_spb + (HV ? 0 : 2);
-    static_assert(HV || 2 * (BLOCK + 2) * sizeof(K) >= 8 * BLOCK, "chunk sources fit the splitter pairs");
     // key-only, per level lev and pair p of the level (slices [p * 2^lev,
     // (p + 1) * 2^lev)), at entry KW - 2 * (KW >> lev) + p: the pair's LDS
     // offset, its second run's offset, its end and the level's co-rank step
@@ -425,8 +422,12 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
             s_o0 = gbase + o;
             s_cnt = bad || (int)p > C ? -1 : (int)c;  // cannot happen with a consistent table
         }
+        // The per-level data below reads the prefix sums this wave has just
+        // written: a wave's LDS accesses complete in order, so no barrier --
+        // only a wavefront-scope fence, which keeps the compiler from moving
+        // a read of another lane's entry above this lane's writes.
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    __syncthreads();
     if (!HV && tid < 64) {
         // wave 0: lane e < KW - 1 builds entry e (level lev, pair p); the
         // level's bound is a max over its pairs, which are an aligned group of
@@ -546,12 +547,17 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
             // and the lanes of each load read their chunk's pair back (one
             // 8-byte read that 64 / IPT lanes share, instead of three
             // ds_bpermute); a descending chunk's cb_lo = B + 1 is odd (B is a
-            // multiple of the item size), so d is its low bit.  Each wave
-            // reads only its own threads' pairs, and a wave's LDS accesses
-            // complete in order.
-            uint2 *s_src2 = (uint2 *)s_spb;
-            s_src2[tid] = make_uint2(cb_lo, cmeta);
-            const uint2 *sq = s_src2 + (tid & ~63) + lane / IPT;
+            // multiple of the item size), so d is its low bit.  The pairs sit
+            // at the start of the wave's own tile range, which no other wave
+            // touches during the load phase and which this wave overwrites
+            // with items only after its last pair read (a wave's LDS accesses
+            // complete in order; the fences keep the compiler from moving a
+            // read of another lane's pair above the writes, or an item write
+            // above the reads).
+            uint2 *s_src2 = (uint2 *)(sk + (tid & ~63) * IPT);
+            s_src2[lane] = make_uint2(cb_lo, cmeta);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const uint2 *sq = s_src2 + lane / IPT;
             const char *gb = (const char *)in + gbase * ES;  // uniform: scalar registers
             const uint32_t rb = (uint32_t)r * ES;
 #pragma unroll
@@ -609,6 +615,7 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
                 vv[i] = real ? xv : 0;
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 #pragma unroll
         for (int i = 0; i < IPT; i++) {
             const int e = lsw<K>(wbase + 64 * (i / NB) + lane) + BLK * (i % NB);
@@ -616,11 +623,14 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
             if constexpr (HV) sv[e] = vv[i];
         }
     }
-    __syncthreads();
+    if constexpr (HV) __syncthreads();
     if constexpr (!HV) {
         // the slices' block splitters: an ascending slice's block is this
         // thread's chunk; an odd slice is stored descending, so the chunk at
-        // c0 is its block (P[j0+1] - c0) / IPT - 1 read backwards
+        // c0 is its block (P[j0+1] - c0) / IPT - 1 read backwards.  A wave's
+        // threads' chunks are the tile range this wave has just written, so a
+        // wavefront fence (no barrier) orders the reads after the writes.
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         if (live) {
             const K first = sk[lsw<K>(c0)], last = sk[lsw<K>(c0 + IPT - 1)];
             if (j0 & 1) {
@@ -688,7 +698,7 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
     }
     // item tid + k * BLOCK sits at LDS index lsw(tid) + k * BLOCK: BLOCK is a
     // multiple of the span the swizzle repeats over
-    static_assert(BLOCK % (sizeof(K) == 4 ? 256 : 128) == 0, "BLOCK: whole swizzle blocks");
+    static_assert(!HV || BLOCK % (sizeof(K) == 4 ? 256 : 128) == 0, "BLOCK: whole swizzle blocks");
     // o0 is the same in every lane: kept in scalar registers, the stores take
     // it as their base and a 32-bit lane offset
     const uint64_t o0u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(o0 >> 32)) << 32) |
