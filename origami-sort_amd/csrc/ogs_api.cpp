@@ -108,11 +108,11 @@ Tuning default_tuning(KeyClass kc) {
     t.v[TP_KWAY_SCORANK] = 1;
     t.v[TP_KWAY_SWHOLE] = 1;
     if (kc == KC_32) {
-        t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 32, 4);
+        t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 32, 6);
         t.v[TP_PLAIN_MERGE] = merge_index(kc, 512, 32);
         t.v[TP_SEG_TINY_BLOCK] = block_index(kc, 64, 16, 1);
         t.v[TP_SEG_SMALL_BLOCK] = block_index(kc, 256, 8, 1);
-        t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 32, 4);
+        t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 32, 6);
         t.v[TP_SEG_MERGE] = merge_index(kc, 256, 16);
         t.v[TP_PLAIN_FANIN] = 2;
         t.v[TP_KWAY_FANIN] = 16;

@@ -881,7 +881,7 @@ static hipError_t launch_quad(const void *in, void *out, uint64_t n, uint64_t R,
 // is the default; the tuning table may pick any other.
 #define OGS_BLOCK_SHAPES_32(X) X(256, 16, 1) X(256, 16, 0) X(256, 16, 2) X(256, 16, 3) X(256, 32, 1) \
     X(512, 16, 1) X(512, 32, 1) X(1024, 16, 1) X(1024, 32, 1) X(128, 16, 1) X(256, 8, 1) X(64, 16, 1) X(64, 8, 1) \
-    X(512, 16, 0) X(1024, 16, 0) X(512, 32, 0) X(512, 32, 2) X(512, 32, 3) X(512, 32, 4) X(512, 16, 3) X(512, 16, 5)
+    X(512, 16, 0) X(1024, 16, 0) X(512, 32, 0) X(512, 32, 2) X(512, 32, 3) X(512, 32, 4) X(512, 16, 3) X(512, 16, 5) X(512, 32, 5) X(512, 32, 6)
 #define OGS_MERGE_SHAPES_32(X) X(256, 16) X(256, 8) X(512, 16) X(128, 16) X(256, 32) X(512, 8) X(1024, 16) \
     X(512, 32) X(128, 4)
 #define OGS_BLOCK_SHAPES_64(X) X(256, 16, 1) X(256, 8, 1) X(128, 16, 1) X(512, 8, 1) X(512, 16, 1) \
