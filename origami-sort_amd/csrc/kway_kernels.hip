@@ -542,18 +542,17 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
         const int wbase = (tid >> 6) * 64 * IPT;
         K kk[IPT];
         uint64_t vv[IPT];
-        if (narrow && !HV) {
-            // key-only: every thread leaves its chunk's (cb_lo, cmeta) in LDS
-            // and the lanes of each load read their chunk's pair back (one
-            // 8-byte read that 64 / IPT lanes share, instead of three
-            // ds_bpermute); a descending chunk's cb_lo = B + 1 is odd (B is a
-            // multiple of the item size), so d is its low bit.  The pairs sit
-            // at the start of the wave's own tile range, which no other wave
-            // touches during the load phase and which this wave overwrites
-            // with items only after its last pair read (a wave's LDS accesses
-            // complete in order; the fences keep the compiler from moving a
-            // read of another lane's pair above the writes, or an item write
-            // above the reads).
+        if (narrow) {
+            // every thread leaves its chunk's (cb_lo, cmeta) in LDS and the
+            // lanes of each load read their chunk's pair back (one 8-byte read
+            // that 64 / IPT lanes share, instead of three ds_bpermute); a
+            // descending chunk's cb_lo = B + 1 is odd (B is a multiple of the
+            // item size), so d is its low bit.  The pairs sit at the start of
+            // the wave's own tile range, which no other wave touches during the
+            // load phase and which this wave overwrites with items only after
+            // its last pair read (a wave's LDS accesses complete in order; the
+            // fences keep the compiler from moving a read of another lane's
+            // pair above the writes, or an item write above the reads).
             uint2 *s_src2 = (uint2 *)(sk + (tid & ~63) * IPT);
             s_src2[lane] = make_uint2(cb_lo, cmeta);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -565,30 +564,18 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
                 // position within the wave's range: BLK * (i % NB) + 64 * (i / NB) + lane
                 const uint2 src = sq[(BLK / IPT) * (i % NB) + (64 / IPT) * (i / NB)];
                 const uint32_t bb = src.x, pm = src.y;
-                const uint32_t d = (uint32_t)__builtin_amdgcn_sbfe((int)bb, 0u, 1u);
-                const int p = __builtin_amdgcn_sbfe((int)pm, (uint32_t)r, 1u);
-                const uint32_t off = ((rb ^ d) + bb) & ~(uint32_t)p;
-                kk[i] = *(const K *)(gb + off) | (K)(typename std::make_signed<K>::type)p;
-            }
-        } else if (narrow) {
-            const char *gb = (const char *)in + gbase * ES;  // uniform: scalar registers
-            const uint32_t rb = (uint32_t)r * ES;
-#pragma unroll
-            for (int i = 0; i < IPT; i++) {
-                const int pw = BLK * (i % NB) + 64 * (i / NB) + lane;  // position within the wave's range
-                const int sl = (pw / IPT) << 2;
-                const uint32_t bb = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_lo);
-                const uint32_t d = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cb_hi);
-                const uint32_t pm = (uint32_t)__builtin_amdgcn_ds_bpermute(sl, (int)cmeta);
                 // -1 for a pad, else 0: a pad reads the group's first item and
                 // becomes the maximum key (value 0)
                 const int p = __builtin_amdgcn_sbfe((int)pm, (uint32_t)r, 1u);
-                const uint32_t off = ((rb ^ d) + bb) & ~(uint32_t)p;
                 if constexpr (HV) {
+                    // KV slices are ascending (d = 0)
+                    const uint32_t off = (rb + bb) & ~(uint32_t)p;
                     const ulonglong2 t = *(const ulonglong2 *)(gb + off);
                     kk[i] = (K)t.x | (K)(int64_t)p;
                     vv[i] = t.y & ~(uint64_t)(int64_t)p;
                 } else {
+                    const uint32_t d = (uint32_t)__builtin_amdgcn_sbfe((int)bb, 0u, 1u);
+                    const uint32_t off = ((rb ^ d) + bb) & ~(uint32_t)p;
                     kk[i] = *(const K *)(gb + off) | (K)(typename std::make_signed<K>::type)p;
                 }
             }
