@@ -672,14 +672,26 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
         }
         __syncthreads();
         if (live) {
-            // the next level's B sides are stored descending (key-only)
-            if (!HV && lev < LOGKW && ((j0 >> lev) & 1)) lds_put_desc<IPT>(sk, end - 1 - diag, r);
-            else lds_put<IPT, HV>(sk, sv, c0, r, v);
             if constexpr (!HV) {
                 // this thread's outputs are block (c0 - pair start) / IPT of
                 // the merged run at the pair's offset: splitter index tid
                 if (lev < LOGKW) put_pair(s_spFL + 2 * tid, r[0], r[IPT - 1]);
             }
+            // the next level's B sides are stored descending (key-only): a
+            // descending thread reverses its registers (exec-masked moves, a
+            // branch that waves holding only ascending runs skip) and writes
+            // them ascending from end - diag - IPT, so a wave holding both
+            // orientations issues one set of 16-byte LDS writes, not two
+            const bool dsc = !HV && lev < LOGKW && ((j0 >> lev) & 1);
+            if (dsc) {
+                asm volatile("");
+                // in place, one v_swap_b32 per pair (the compiler otherwise
+                // copies through temporaries, or pairs the moves into packed
+                // 64-bit moves)
+#pragma unroll
+                for (int i = 0; i < IPT / 2; i++) swap_regs(r[i], r[IPT - 1 - i]);
+            }
+            lds_put<IPT, HV>(sk, sv, dsc ? end - diag - IPT : c0, r, v);
         }
         __syncthreads();
     }

@@ -573,10 +573,29 @@ __device__ __forceinline__ int lds_corank_split_rev(const K *sk, const K *spFL, 
         const K kb = *(const K *)(sb + (swb ^ (tb & MW)));
         vb = kb < ka ? vb : tb;
     }
-    return a0 + vb / (int)W + 1;
+    // vb is a multiple of W, at least -W: the arithmetic shift divides exactly
+    return a0 + (vb >> __builtin_ctz(W)) + 1;
 }
 
 // A block's splitter pair (its first and last key) as one 8- or 16-byte store.
+// Exchanges two registers in place with v_swap_b32 (one per 32-bit half).
+template <typename K> __device__ __forceinline__ void swap_regs(K &a, K &b) {
+    static_assert(sizeof(K) == 4 || sizeof(K) == 8, "32- or 64-bit keys");
+    if constexpr (sizeof(K) == 4) {
+        uint32_t x = (uint32_t)a, y = (uint32_t)b;
+        asm("v_swap_b32 %0, %1" : "+v"(x), "+v"(y));
+        a = (K)x;
+        b = (K)y;
+    } else {
+        uint32_t xl = (uint32_t)a, xh = (uint32_t)((uint64_t)a >> 32);
+        uint32_t yl = (uint32_t)b, yh = (uint32_t)((uint64_t)b >> 32);
+        asm("v_swap_b32 %0, %1" : "+v"(xl), "+v"(yl));
+        asm("v_swap_b32 %0, %1" : "+v"(xh), "+v"(yh));
+        a = (K)(((uint64_t)xh << 32) | xl);
+        b = (K)(((uint64_t)yh << 32) | yl);
+    }
+}
+
 template <typename K> __device__ __forceinline__ void put_pair(K *p, K first, K last) {
     if constexpr (sizeof(K) == 4) *(uint2 *)p = make_uint2(first, last);
     else *(ulonglong2 *)p = make_ulonglong2((unsigned long long)first, (unsigned long long)last);
