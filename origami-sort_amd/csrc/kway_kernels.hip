@@ -527,7 +527,110 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
             cmeta = (uint32_t)(v0 | (v1 << 8) | ((int)desc << 16));
         }
     }
-    {
+    const uint64_t gspan = (gend - gbase) * ES;  // the group's bytes
+    const bool vec16 = !HV && narrow && gspan >= 16 && gspan <= (1ull << 31);  // uniform
+    if constexpr (!HV) {
+        if (vec16) {
+            // Key-only groups of 16 bytes to 2 GiB: 16-byte loads.  Wave w
+            // loads LDS positions [w * 64 * IPT, (w + 1) * 64 * IPT); load c
+            // of a lane covers the E = 16 / ES positions wbase + 64 * E * c +
+            // E * lane .. + E - 1 (lane-consecutive 16-byte windows), which lie
+            // in the chunk of wave lane (64 * E * c + E * lane) / IPT: that
+            // lane's (cb_lo, cmeta) pair is read back from LDS as in the item
+            // form below.  An ascending chunk's window is the E items from its
+            // first position's item; a descending chunk's is the E items below
+            // and including it, reversed in registers.  A window of pads only
+            // reads the group's first 16 bytes.  A window that is not inside
+            // the group (at most one per slice end, or its first or last
+            // items) loads its real items one at a time, its pads reading the
+            // group's first item; pads become the maximum key.  The group's
+            // byte span is at most 2^31, so a window starting below the group
+            // (a descending chunk at its first items) wraps to an offset above
+            // the limit.
+            constexpr int E = chunk_items<K>();
+            constexpr int NL = IPT / E;         // 16-byte loads per thread
+            constexpr int CPL = 64 * E / IPT;   // chunks per load of a wave
+            static_assert(IPT % E == 0 && (64 * E) % IPT == 0, "whole 16-byte windows, whole chunks per load");
+            typedef uint32_t u32x4a __attribute__((ext_vector_type(4), aligned(4)));
+            typedef uint64_t u64x2a __attribute__((ext_vector_type(2), aligned(8)));
+            const int wbase = (tid >> 6) * 64 * IPT;
+            uint2 *s_src2 = (uint2 *)(sk + wbase);
+            s_src2[lane] = make_uint2(cb_lo, cmeta);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            const uint2 *sq = s_src2 + (E * lane) / IPT;
+            const char *gb = (const char *)in + gbase * ES;  // uniform: scalar registers
+            const int r0 = (E * lane) & (IPT - 1);
+            const uint32_t rb = (uint32_t)r0 * ES;
+            const uint32_t lim = (uint32_t)(gspan - 16);
+            constexpr uint32_t WIN = (uint32_t)((E - 1) * ES);
+            constexpr uint32_t EM = (1u << E) - 1;
+            K x[NL][E];
+            uint32_t tb[NL];
+#pragma unroll
+            for (int c = 0; c < NL; c++) {
+                const uint2 src = sq[CPL * c];
+                const uint32_t bb = src.x;
+                const uint32_t t = src.y >> r0;  // bit j: position r0 + j is a pad
+                // 0 ascending, ~0 descending (bb = B + 1, odd)
+                const uint32_t d = (uint32_t)__builtin_amdgcn_sbfe((int)bb, 0u, 1u);
+                // ascending: B + r0 * ES; descending: B - r0 * ES - WIN
+                const uint32_t start = ((rb ^ d) + bb) - (d & WIN);
+                const uint32_t off = (t & EM) == EM ? 0u : start;
+                tb[c] = (t & EM) | (d & 0x80000000u);  // the window's pad bits; the top bit: descending
+                if (off <= lim) {
+                    if constexpr (E == 4) {
+                        const u32x4a q = *(const u32x4a *)(gb + off);
+                        x[c][0] = (K)q.x; x[c][1] = (K)q.y; x[c][2] = (K)q.z; x[c][3] = (K)q.w;
+                    } else {
+                        const u64x2a q = *(const u64x2a *)(gb + off);
+                        x[c][0] = (K)q.x; x[c][1] = (K)q.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < E; i++) {
+                        // memory item i of the window is position r0 + (d ? E - 1 - i : i)
+                        const int j = d ? E - 1 - i : i;
+                        const bool pad = (t >> j) & 1u;
+                        x[c][i] = *(const K *)(gb + (pad ? 0u : start + (uint32_t)i * ES));
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < NL; c++) {
+                if ((int)tb[c] < 0) {
+                    // descending: memory order reversed into position order
+                    asm volatile("");
+#pragma unroll
+                    for (int i = 0; i < E / 2; i++) swap_regs(x[c][i], x[c][E - 1 - i]);
+                }
+#pragma unroll
+                for (int i = 0; i < E; i++) {
+                    // the builtin returns an unsigned int: through int, so the
+                    // pad mask sign-extends to 64-bit keys
+                    const int p = (int)__builtin_amdgcn_sbfe((int)tb[c], (uint32_t)i, 1u);
+                    x[c][i] |= (K)(typename std::make_signed<K>::type)p;
+                }
+            }
+            // every pair read of this wave precedes the item writes that
+            // overwrite the pairs
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // window c of the lane sits at lds_off(wbase + E * lane) ^
+            // lds_off(64 * E * c) = + 1024 * c bytes (XOR-linear swizzle,
+            // disjoint bits)
+            char *wp = (char *)sk + lds_off<K>(wbase + E * lane);
+#pragma unroll
+            for (int c = 0; c < NL; c++) {
+                if constexpr (E == 4) {
+                    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                    *(u32x4 *)(wp + 1024 * c) = u32x4{(uint32_t)x[c][0], (uint32_t)x[c][1], (uint32_t)x[c][2], (uint32_t)x[c][3]};
+                } else {
+                    typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
+                    *(u64x2 *)(wp + 1024 * c) = u64x2{(uint64_t)x[c][0], (uint64_t)x[c][1]};
+                }
+            }
+        }
+    }
+    if (HV || !vec16) {
         // Wave w loads LDS positions [w * 64 * IPT, (w + 1) * 64 * IPT), the
         // chunks of its own lanes: load i covers positions wbase + BLK * (i %
         // NB) + 64 * (i / NB) + lane (BLK = the index span the LDS swizzle
