@@ -119,11 +119,11 @@ Tuning default_tuning(KeyClass kc) {
         t.v[TP_KWAY_SHAPE] = kway_index(kc, 512, 16);
         t.v[TP_KWAY_SAMPLE] = 6;
     } else if (kc == KC_64) {
-        t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 16, 2);
+        t.v[TP_PLAIN_BLOCK] = block_index(kc, 512, 16, 3);
         t.v[TP_PLAIN_MERGE] = merge_index(kc, 512, 16);
         t.v[TP_SEG_TINY_BLOCK] = block_index(kc, 64, 8, 1);
         t.v[TP_SEG_SMALL_BLOCK] = block_index(kc, 256, 8, 1);
-        t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 16, 2);
+        t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 16, 3);
         t.v[TP_SEG_MERGE] = merge_index(kc, 256, 16);
         t.v[TP_PLAIN_FANIN] = 2;
         t.v[TP_KWAY_FANIN] = 16;
@@ -137,8 +137,8 @@ Tuning default_tuning(KeyClass kc) {
         t.v[TP_SEG_LARGE_BLOCK] = block_index(kc, 512, 8, 3);
         t.v[TP_SEG_MERGE] = merge_index(kc, 512, 4);
         t.v[TP_PLAIN_FANIN] = 4;
-        t.v[TP_KWAY_FANIN] = 8;
-        t.v[TP_KWAY_SHAPE] = kway_index(kc, 512, 4);
+        t.v[TP_KWAY_FANIN] = 16;
+        t.v[TP_KWAY_SHAPE] = kway_index(kc, 1024, 4);
         t.v[TP_KWAY_SAMPLE] = 5;
     }
     return t;
