@@ -490,19 +490,21 @@ def main():
     def step():
         return ctx.sort_keep_ptr(kind, d_in.data_ptr(), d_a.data_ptr(), d_b.data_ptr(), n)
 
-    log(f"bench: warmup {args.warmup}")
-    res = None
-    for _ in range(args.warmup):
-        res = step()
+    # size-independent checks of one sort's result, before the warmup steps
+    # (so the timed steps follow the warmup directly): sorted in the key
+    # order, and the same multiset of items (wrapping sum, xor and sum of
+    # squares of the raw bytes as 64-bit words; for 4-byte keys, of the keys)
+    res = step()
     torch.cuda.synchronize()
-    # size-independent checks of the result: sorted in the key order, and the
-    # same multiset of items (wrapping sum, xor and sum of squares of the raw
-    # bytes as 64-bit words; for 4-byte keys, of the keys)
     out_t = d_a if res == d_a.data_ptr() else d_b
     ok_sorted, ok_multiset = result_checks(kind, d_in, out_t, n)
     log(f"bench: result sorted={ok_sorted} multiset={ok_multiset}")
     assert ok_sorted and ok_multiset
+    del out_t
 
+    log(f"bench: warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
     torch.cuda.synchronize()
     log(f"bench: timing {args.steps} steps")
     t0 = time.perf_counter()
