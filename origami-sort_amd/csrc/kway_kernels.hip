@@ -603,12 +603,17 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
 #pragma unroll
                     for (int i = 0; i < E / 2; i++) swap_regs(x[c][i], x[c][E - 1 - i]);
                 }
+                if (tb[c] & EM) {
+                    // a window with pads (at most two per slice: a branch that
+                    // most waves skip)
+                    asm volatile("");
 #pragma unroll
-                for (int i = 0; i < E; i++) {
-                    // the builtin returns an unsigned int: through int, so the
-                    // pad mask sign-extends to 64-bit keys
-                    const int p = (int)__builtin_amdgcn_sbfe((int)tb[c], (uint32_t)i, 1u);
-                    x[c][i] |= (K)(typename std::make_signed<K>::type)p;
+                    for (int i = 0; i < E; i++) {
+                        // the builtin returns an unsigned int: through int, so
+                        // the pad mask sign-extends to 64-bit keys
+                        const int p = (int)__builtin_amdgcn_sbfe((int)tb[c], (uint32_t)i, 1u);
+                        x[c][i] |= (K)(typename std::make_signed<K>::type)p;
+                    }
                 }
             }
             // every pair read of this wave precedes the item writes that
