@@ -62,6 +62,25 @@ __host__ __device__ inline uint64_t kw_last_samples(uint64_t n, int rlog2, int s
     return ns;
 }
 
+// Division of a tile index t < 2^32 by a tiles-per-group count d (1 <= d <
+// 2^32) with m = min(floor(2^32 / d), 2^32 - 1), computed once on the host:
+// mulhi(t, m) is the quotient or one below it, so one correction step makes it
+// exact.  With t, d and m uniform this is scalar code; the compiler's own
+// division is about 20 vector instructions.
+__host__ __device__ inline uint32_t kw_div_magic(uint64_t d) {
+    const uint64_t m = (1ull << 32) / d;
+    return m > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)m;
+}
+__device__ __forceinline__ uint32_t kw_div(uint32_t t, uint32_t d, uint32_t m, uint32_t &r) {
+    uint32_t q = __umulhi(t, m);
+    r = t - q * d;
+    if (r >= d) {
+        q++;
+        r -= d;
+    }
+    return q;
+}
+
 uint64_t kway_tiles(uint64_t n, int rlog2, int slog2, int kw, uint64_t tile, int ipt) {
     if (n == 0) return 0;
     const uint64_t q = kw_q(tile, ipt, kw, slog2);
@@ -353,9 +372,9 @@ __device__ __forceinline__ void store_chunks(const K *sk, K *ob, int cnt, int ti
 // workgroup).
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
 __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
-                                                int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
-                                                const uint64_t *__restrict__ tab, const SampleSpec &sp,
-                                                uint32_t *__restrict__ err, const uint64_t t) {
+                                                int rlog2, uint64_t ntg, uint32_t ntg_m, uint64_t nt_last,
+                                                int cout, const uint64_t *__restrict__ tab, const SampleSpec &sp,
+                                                uint32_t *__restrict__ err, const uint32_t t) {
     static_assert(KW == 2 || KW == 4 || KW == 8 || KW == 16 || KW == 32, "fan-in 2, 4, 8, 16 or 32");
     static_assert((IPT & (IPT - 1)) == 0 && IPT >= 4, "IPT: a power of two >= 4");
     constexpr int C = BLOCK * IPT;
@@ -386,7 +405,8 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
     uint64_t *sv = svb + (HV ? lds_front<K>() : 0);
     const int tid = threadIdx.x, lane = tid & 63;
     const uint64_t R = 1ull << rlog2;
-    const uint64_t g = t / ntg, tl = t % ntg;
+    uint32_t tl32;
+    const uint64_t g = kw_div(t, (uint32_t)ntg, ntg_m, tl32), tl = tl32;
     const uint64_t gbase = g * KW * R;
     const uint64_t nt_here = gbase + KW * R >= n ? nt_last : ntg;
     if (tid < 64) {
@@ -836,10 +856,10 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
 
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_kway_merge(const void *__restrict__ in, void *__restrict__ out, uint64_t n,
-                                                      int rlog2, uint64_t ntg, uint64_t nt_last, int cout,
-                                                      const uint64_t *__restrict__ tab, SampleSpec sp,
+                                                      int rlog2, uint64_t ntg, uint32_t ntg_m, uint64_t nt_last,
+                                                      int cout, const uint64_t *__restrict__ tab, SampleSpec sp,
                                                       uint32_t *__restrict__ err) {
-    kway_merge_tile<K, HV, KW, BLOCK, IPT>(in, out, n, rlog2, ntg, nt_last, cout, tab, sp, err, blockIdx.x);
+    kway_merge_tile<K, HV, KW, BLOCK, IPT>(in, out, n, rlog2, ntg, ntg_m, nt_last, cout, tab, sp, err, blockIdx.x);
 }
 
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
@@ -860,7 +880,7 @@ static hipError_t launch_kway(const void *in, void *out, uint64_t n, int rlog2, 
     if (mid) (void)hipEventRecord(mid, s);
     const SampleSpec none = {nullptr, 0, 0, 0};
     hipLaunchKernelGGL((k_kway_merge<K, HV, KW, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, in, out, n,
-                       rlog2, ntg, nt_last, cout, (const uint64_t *)tab, next ? *next : none, err);
+                       rlog2, ntg, kw_div_magic(ntg), nt_last, cout, (const uint64_t *)tab, next ? *next : none, err);
     return hipGetLastError();
 }
 
@@ -877,7 +897,7 @@ static hipError_t launch_kway_whole(const void *in, void *out, uint64_t n, int r
     const uint64_t groups = (n + R * KW - 1) / (R * KW);
     const SampleSpec none = {nullptr, 0, 0, 0};
     hipLaunchKernelGGL((k_kway_merge<K, HV, KW, BLOCK, IPT>), dim3((unsigned)groups), dim3(BLOCK), 0, s, in, out, n,
-                       rlog2, (uint64_t)1, (uint64_t)1, 0, (const uint64_t *)nullptr, none, err);
+                       rlog2, (uint64_t)1, kw_div_magic(1), (uint64_t)1, 0, (const uint64_t *)nullptr, none, err);
     return hipGetLastError();
 }
 
@@ -921,7 +941,7 @@ __global__ __launch_bounds__(256) void k_seg_kway_bounds(const KSegH *__restrict
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
 __global__ __launch_bounds__(BLOCK) void k_seg_kway_merge(const KSegH *__restrict__ segs,
                                                           const uint2 *__restrict__ tmap, int pass, int rlog2,
-                                                          int slog2, uint64_t Q, uint64_t ntg,
+                                                          int slog2, uint64_t Q, uint64_t ntg, uint32_t ntg_m,
                                                           uint32_t *__restrict__ err) {
     const uint2 m = tmap[blockIdx.x];
     const KSegH &g = segs[m.x];
@@ -929,7 +949,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg_kway_merge(const KSegH *__restric
     const bool last = pass + 1 >= (int)g.passes;
     SampleSpec sp = {nullptr, 0, 0, 0};
     if (!last) sp = {g.smp[0], rlog2 + (int)g.lv[pass], 1 << g.lv[pass + 1], (int)g.sl[pass + 1]};
-    kway_merge_tile<K, HV, KW, BLOCK, IPT>(g.buf[pass & 1], g.buf[(pass + 1) & 1], g.len, rlog2, ntg, nt_last,
+    kway_merge_tile<K, HV, KW, BLOCK, IPT>(g.buf[pass & 1], g.buf[(pass + 1) & 1], g.len, rlog2, ntg, ntg_m, nt_last,
                                            last ? (int)g.cout : 0, g.tab, sp, err, m.y);
 }
 
@@ -945,7 +965,7 @@ static hipError_t launch_seg_kway(const KSegH *segs, const uint2 *tmap, uint64_t
     hipLaunchKernelGGL((k_seg_kway_bounds<K, HV, KW>), dim3((unsigned)((ntiles + tpb - 1) / tpb)), dim3(256), 0, s,
                        segs, tmap, ntiles, pass, rlog2, slog2, Q, ntg, smi);
     hipLaunchKernelGGL((k_seg_kway_merge<K, HV, KW, BLOCK, IPT>), dim3((unsigned)ntiles), dim3(BLOCK), 0, s, segs,
-                       tmap, pass, rlog2, slog2, Q, ntg, err);
+                       tmap, pass, rlog2, slog2, Q, ntg, kw_div_magic(ntg), err);
     return hipGetLastError();
 }
 
@@ -963,7 +983,8 @@ __global__ __launch_bounds__(BLOCK) void k_seg_kway_whole(const KSegH *__restric
     const int sl = (int)g.sl[pass];
     const uint64_t ns = (g.len + (1ull << sl) - 1) >> sl;
     const SampleSpec none = {nullptr, 0, 0, 0};
-    kway_merge_tile<K, HV, KW, BLOCK, IPT>(g.smp[0], g.smp[1], ns, rlog2, 1, 1, 0, nullptr, none, err, m.y);
+    kway_merge_tile<K, HV, KW, BLOCK, IPT>(g.smp[0], g.smp[1], ns, rlog2, 1, kw_div_magic(1), 1, 0, nullptr, none, err,
+                                           m.y);
 }
 
 template <typename K, bool HV, int KW, int BLOCK, int IPT>
