@@ -410,7 +410,9 @@ def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5, coalesce=0):
             "serial_time_by_size": buckets,
             "serial_wall_ms": round(sum(svc) * 1e3, 3), "serial_kernel_ms": round(sum(ker) * 1e3, 3),
             "coalesced_bound": {"ms": round(bound_s * 1e3, 3), "keys_per_s": round(total_keys / bound_s / 1e9, 4),
+                                "vs_serial": round(span_s / bound_s, 3),
                                 "vs_serial_wall": round(sum(svc) / bound_s, 3),
+                                "server_vs_bound": round(bound_s / span_v, 3),
                                 "note": "each key type's requests in one ogs_sort_batch, types back to back"},
             "streams": streams, "unit": "G keys/s", "identical_results": bool(same),
             "note": "device-resident requests; latency = submit (arrival) to result ready"}
