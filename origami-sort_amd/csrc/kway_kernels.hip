@@ -791,10 +791,16 @@ __device__ __forceinline__ void kway_merge_tile(const void *__restrict__ in, voi
                     if (c == IPT) lds_merge_serial<IPT, true, true>(sk, sv, a_off, la, mid, lb, a, diag - a, 0, IPT, r, v);
                     else lds_merge_serial<IPT, true, false>(sk, sv, a_off, la, mid, lb, a, diag - a, 0, c, r, v);
                 }
+                if (c < IPT) {
+                    // outputs past the pair's real items are pads (threads at
+                    // a pair's real end or in its pad region: a branch most
+                    // waves skip)
+                    asm volatile("");
 #pragma unroll
-                for (int i = 0; i < IPT; i++) {
-                    r[i] = i < c ? r[i] : KMAX;
-                    v[i] = i < c ? v[i] : 0;
+                    for (int i = 0; i < IPT; i++) {
+                        r[i] = i < c ? r[i] : KMAX;
+                        v[i] = i < c ? v[i] : 0;
+                    }
                 }
             }
         }
