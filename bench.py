@@ -421,9 +421,9 @@ def bench_serving(ctx, nreq, rate_factor, streams=4, seed=5, coalesce=0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--profile-steps", type=int, default=0,
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=10,
                     help="sorts run with per-kernel HIP events after the timed region (0: --steps)")
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--kind", default="u32", choices=["u32", "f32", "u64", "kv_u64u64"])
