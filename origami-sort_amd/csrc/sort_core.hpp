@@ -222,10 +222,11 @@ __device__ __forceinline__ void lane_pair_keep(K &mine, uint64_t &vm, K other, u
         mine = lo < t ? t : lo;
         (void)vm; (void)vo;
     } else if constexpr (!HV) {
-        // key-only: equal keys are identical bits, so the pair is min / max
-        const K lo = other < mine ? other : mine;
-        const K hi = other < mine ? mine : other;
-        mine = upper ? hi : lo;
+        // key-only 64-bit: one compare; a lane takes its partner's item when
+        // that is smaller (lower lane) or not smaller (upper lane) -- equal
+        // keys are identical bits, so either choice keeps every item once
+        const bool take = (other < mine) != upper;
+        mine = take ? other : mine;
         (void)vm; (void)vo;
     } else {
         bool keep = upper ? !(mine < other) : !(other < mine);
