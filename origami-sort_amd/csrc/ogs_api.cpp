@@ -722,7 +722,10 @@ int Ctx::batch_build(ogs_key_t kt, const SegSpec *segs, size_t nseg, bool allow_
                     W.pass = p;
                     W.lv = lvv;
                     W.smi = 1;
-                    W.whole = kwh;
+                    // the smallest shape that holds this fan-in's groups (every
+                    // fan-in up to lvmax fits, as the largest does)
+                    W.whole = whole_group_shape(b.skc, 1 << lvv, rl - sl);
+                    if (!W.whole) W.whole = kwh;
                     W.tfirst.push_back(0);
                     const uint64_t gs = 1ull << (rl - sl + lvv);  // sample records per group
                     for (size_t i = 0; i < b.kseg.size(); i++) {
