@@ -51,7 +51,10 @@ def check_u32(src, out, n):
     assert (q_in - q_out) % (1 << 64) == 0, "sum of squares differs"
 
 
-@pytest.mark.parametrize("n", [(1 << 30) + 17, (1 << 32) + 5])
+# (1 << 29) + 7: the last K-way pass merges one group of more than 2 GiB (the
+# per-item load path); (1 << 30) + 17: a group of more than 4 GiB (64-bit
+# chunk sources); (1 << 32) + 5: beyond 32-bit item counts
+@pytest.mark.parametrize("n", [(1 << 29) + 7, (1 << 30) + 17, (1 << 32) + 5])
 def test_u32_large(ctx, n):
     import torch
     need = 3 * 4 * n + (2 << 30)
@@ -71,6 +74,46 @@ def test_u32_large(ctx, n):
     p = ctx.sort_keep_ptr(ogs.U32, src.data_ptr(), a.data_ptr(), b.data_ptr(), n)
     torch.cuda.synchronize()
     check_u32(src, a if p == a.data_ptr() else b, n)
+
+
+def test_u64_large(ctx):
+    """2^28 + 3 u64 keys: the last K-way pass merges one group of more than
+    2 GiB, whose chunks load item by item."""
+    import torch
+    n = (1 << 28) + 3
+    need = 3 * 8 * n + (2 << 30)
+    free, _ = torch.cuda.mem_get_info()
+    if free < need:
+        pytest.skip(f"needs {need >> 30} GiB of free HBM")
+    say(f"case large u64 n={n}")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    src = torch.empty(n, dtype=torch.int64, device="cuda")
+    for o in range(0, n, CHUNK):
+        c = min(n, o + CHUNK) - o
+        src[o:o + c] = torch.randint(-(1 << 63), (1 << 63) - 1, (c,), dtype=torch.int64, device="cuda", generator=g)
+    a = torch.empty_like(src)
+    b = torch.empty_like(src)
+    p = ctx.sort_keep_ptr(ogs.U64, src.data_ptr(), a.data_ptr(), b.data_ptr(), n)
+    torch.cuda.synchronize()
+    out = a if p == a.data_ptr() else b
+    flip = -(1 << 63)
+    s_in = s_out = q_in = q_out = 0
+    prev = None
+    for o in range(0, n, CHUNK):
+        x = src[o:min(n, o + CHUNK)]
+        k = out[o:min(n, o + CHUNK)] ^ flip  # unsigned order as signed
+        assert bool((k[1:] >= k[:-1]).all()), f"not sorted inside chunk at {o}"
+        if prev is not None:
+            assert int(k[0]) >= prev, f"not sorted across chunk edge at {o}"
+        prev = int(k[-1])
+        y = out[o:min(n, o + CHUNK)]
+        # wrapping 64-bit sums of the items and of their squares
+        s_in += int(x.sum()); s_out += int(y.sum())
+        q_in += int((x * x).sum()); q_out += int((y * y).sum())
+        del x, y, k
+    assert (s_in - s_out) % (1 << 64) == 0, "sum differs"
+    assert (q_in - q_out) % (1 << 64) == 0, "sum of squares differs"
 
 
 def test_kv_large(ctx):
